@@ -1,0 +1,143 @@
+/*
+ * strassen_amd.h -- C ABI of the MI355X (gfx950) block-sparse multiply library.
+ *
+ * The reference (diersam/Strassen) is a header-only C++ template library with
+ * no FFI; its callers include blocksparsematrix.h / strassen.hpp directly.
+ * This ABI is what those headers' bodies call in the drop-in C++ layer
+ * (strassen_amd/cpp/): every entry point below names the reference interface
+ * it replaces.  Plain pointers and sizes only; no C++ types cross it.
+ *
+ * Conventions
+ *  - return value: SA_OK (0) or a negative SA_ERR_* code; sa_last_error()
+ *    gives the message of the calling thread's last failure;
+ *  - host buffers are caller-owned and column-major; device memory is owned by
+ *    the context / matrices;
+ *  - a context owns one HIP stream; all calls on a context are ordered on it
+ *    and calls that return host data synchronise it;
+ *  - a block-sparse matrix (sa_bsm) is a grid of column-major blocks, grid
+ *    index rb + cb*nrb (blocksparsematrix.h:71-72), each block with a cached
+ *    Frobenius norm that follows the reference's staleness rules.
+ */
+#ifndef STRASSEN_AMD_H
+#define STRASSEN_AMD_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SA_OK 0
+#define SA_ERR_INVALID (-1)     /* bad argument, shape or block geometry */
+#define SA_ERR_HIP (-2)         /* HIP runtime failure */
+#define SA_ERR_OOM (-3)         /* device allocation failed */
+#define SA_ERR_UNSUPPORTED (-4) /* valid request outside what this build supports */
+
+typedef struct sa_context* sa_ctx_t;
+typedef struct sa_bsm* sa_bsm_t;
+
+/* Multiply options (no counterpart in the reference; defaults = reference
+ * semantics in FP64). */
+#define SA_MODE_FP64 0
+#define SA_MODE_OZAKI 1
+#define SA_SLICE_FP16 0
+#define SA_SLICE_BF16 1
+#define SA_SCHED_HEURISTIC 0 /* dry-run cost model, strassen.hpp:579-673 */
+#define SA_SCHED_MLP 1       /* learned per-node selector, heuristic as fallback */
+#define SA_SCHED_TREE 2      /* caller-supplied decision tree bits */
+typedef struct {
+  int32_t mode;        /* SA_MODE_* */
+  int32_t slices;      /* Ozaki: number of slices per operand (2..6) */
+  int32_t slice_type;  /* SA_SLICE_* */
+  int32_t tile;        /* GEMM tile config id, -1 = autotuned choice */
+  int32_t scheduler;   /* SA_SCHED_* (hybrid Strassen only) */
+  int32_t reserved;
+  int64_t cutoff;      /* min_size_for_strassen (main.cpp:9), elements */
+  const uint8_t* tree; /* SA_SCHED_TREE: bits in dry-run layout */
+} sa_opts_t;
+
+void sa_opts_default(sa_opts_t* o);
+
+/* ------------------------------------------------------------ context -- */
+/* Last error message of the calling thread ("" if none). */
+const char* sa_last_error(void);
+int sa_device_count(int* n);
+/* Context on one device: stream, workspace, profiling counters. */
+int sa_ctx_create(int device, sa_ctx_t* out);
+int sa_ctx_destroy(sa_ctx_t ctx);
+int sa_ctx_synchronize(sa_ctx_t ctx);
+/* Profiling: when on, every launch of the dominant kernels is bracketed by
+ * HIP events on the context stream; sa_ctx_profile returns the accumulated
+ * kernel time (ms) and launch count of one kernel class
+ * ("gemm_fp64", "gemm_ozaki", "screen", "norms", "winograd", ...). */
+int sa_ctx_set_profiling(sa_ctx_t ctx, int on);
+int sa_ctx_profile(sa_ctx_t ctx, const char* kernel_class, double* ms, int64_t* launches);
+int sa_ctx_profile_reset(sa_ctx_t ctx);
+/* Wall time of the context stream between two sa_ctx_mark calls (ms). */
+int sa_ctx_mark(sa_ctx_t ctx, int slot);
+int sa_ctx_elapsed(sa_ctx_t ctx, int slot0, int slot1, float* ms);
+
+/* ---------------------------------------------------- block-sparse matrix -- */
+/* BlockSparseMatrix(nr, nc, bs_r, bs_c, thresh) -- blocksparsematrix.hpp:6-16
+ * (uniform target block size, ragged last block row/column). */
+int sa_bsm_create(sa_ctx_t ctx, int64_t nrow, int64_t ncol, int32_t bs_row, int32_t bs_col,
+                  double thresh, sa_bsm_t* out);
+/* Extension: uneven block rows/columns (actual sizes rsz/csz; screening sizes
+ * rscr/cscr, NULL = actual sizes). */
+int sa_bsm_create_blocked(sa_ctx_t ctx, int32_t nrb, const int32_t* rsz, const int32_t* rscr,
+                          int32_t ncb, const int32_t* csz, const int32_t* cscr, double thresh,
+                          sa_bsm_t* out);
+int sa_bsm_destroy(sa_bsm_t m);
+/* nrow(), ncol(), nrowblocks(), ncolblocks(), no_of_alloc_blocks()
+ * (blocksparsematrix.h:49-58); arena bytes in use. */
+int sa_bsm_info(sa_bsm_t m, int64_t* nrow, int64_t* ncol, int32_t* nrb, int32_t* ncb,
+                int64_t* nalloc, int64_t* arena_bytes);
+/* Copy assignment / copy ctor (blocks and cached norms). */
+int sa_bsm_copy(sa_bsm_t dst, sa_bsm_t src);
+/* copy_from_input_array (blocksparsematrix.hpp:35-61): blocks with
+ * norm < thresh*bs_r*bs_c are dropped. */
+int sa_bsm_from_dense(sa_bsm_t m, const double* host, int64_t ld);
+/* to_pointer (blocksparsematrix.hpp:103-124): writes present blocks only. */
+int sa_bsm_to_dense(sa_bsm_t m, double* host, int64_t ld);
+/* Block presence (1/0) and cached norms, grid order. Either may be NULL. */
+int sa_bsm_get_meta(sa_bsm_t m, uint8_t* present, double* norms);
+/* Extension: synthetic input generated in HBM.  Block b gets
+ * block_scale[b] * z(seed, global element index) (z defined in
+ * strassen_amd/synth.py), then the copy-in drop rule applies, exactly as
+ * sa_bsm_from_dense of the equivalent dense matrix. */
+int sa_bsm_generate(sa_bsm_t m, uint64_t seed, const double* block_scale);
+/* fill_with_values (blocksparsematrix.hpp:84-101): all blocks, norm sqrt(r*c)*v. */
+int sa_bsm_fill(sa_bsm_t m, double v);
+/* zero() (blocksparsematrix.hpp:79-82): all blocks empty. */
+int sa_bsm_zero(sa_bsm_t m);
+/* scale / *= (blocksparsematrix.hpp:71-77, 250-254): cached norm *= s*s. */
+int sa_bsm_scale(sa_bsm_t m, double s);
+/* calc_frobenius_norms (blocksparsematrix.hpp:139-147). */
+int sa_bsm_calc_norms(sa_bsm_t m);
+/* recompress (blocksparsematrix.hpp:149-156). */
+int sa_bsm_recompress(sa_bsm_t m);
+/* operator+= (sign>0, :170-194) / operator-= (sign<0, :196-221). */
+int sa_bsm_add(sa_bsm_t m, sa_bsm_t rhs, double sign);
+/* add_transpose (blocksparsematrix.hpp:224-248). */
+int sa_bsm_add_transpose(sa_bsm_t m, sa_bsm_t rhs);
+/* frobenius_norm (blocksparsematrix.hpp:256-261). */
+int sa_bsm_frobenius_norm(sa_bsm_t m, double* out);
+/* dot(lhs, rhs, thresh) (blocksparsematrix.hpp:263-290). */
+int sa_bsm_dot(sa_bsm_t a, sa_bsm_t b, double thresh, double* out);
+
+/* ------------------------------------------------------------- multiply -- */
+/* matmult(C, A, transA, B, transB, thresh, alpha, beta)
+ * -- blocksparsematrix.h:95-99 / blocksparsematrix.hpp:292-414.
+ * opts may be NULL (FP64). */
+int sa_matmult(sa_bsm_t C, sa_bsm_t A, int transA, sa_bsm_t B, int transB, double thresh,
+               double alpha, double beta, const sa_opts_t* opts);
+/* Screened task list of the context's last sa_matmult / hybrid leaf set:
+ * (ib, jb, kb) int32 triples in canonical order (C blocks column-major, kb
+ * ascending).  *n receives the full count (the reference's nsig,
+ * blocksparsematrix.hpp:353,402). */
+int sa_tasklist_get(sa_ctx_t ctx, int32_t* triples, int64_t cap, int64_t* n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

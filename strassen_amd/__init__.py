@@ -1,0 +1,277 @@
+"""strassen_amd -- MI355X block-sparse multiply (diersam/Strassen hot path).
+
+This module is the Python view of the C ABI declared in include/strassen_amd.h
+(used by the tests and bench.py).  It loads the in-tree HIP library
+strassen_amd/lib/libstrassen_amd.so and fails loudly when it is missing: there
+is no host fallback.  The C++ drop-in layer for callers of the reference's
+headers lives in strassen_amd/cpp/.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "lib", "libstrassen_amd.so")
+
+SA_OK = 0
+MODE_FP64, MODE_OZAKI = 0, 1
+SLICE_FP16, SLICE_BF16 = 0, 1
+SCHED_HEURISTIC, SCHED_MLP, SCHED_TREE = 0, 1, 2
+
+_lib = None
+P = C.c_void_p
+I32 = C.c_int32
+I64 = C.c_int64
+D = C.c_double
+INT = C.c_int
+
+
+class Opts(C.Structure):
+    _fields_ = [("mode", I32), ("slices", I32), ("slice_type", I32), ("tile", I32),
+                ("scheduler", I32), ("reserved", I32), ("cutoff", I64), ("tree", P)]
+
+
+# name -> (restype, argtypes); every function declared in include/strassen_amd.h
+ABI = {
+    "sa_opts_default": (None, [P]),
+    "sa_last_error": (C.c_char_p, []),
+    "sa_device_count": (INT, [P]),
+    "sa_ctx_create": (INT, [INT, P]),
+    "sa_ctx_destroy": (INT, [P]),
+    "sa_ctx_synchronize": (INT, [P]),
+    "sa_ctx_set_profiling": (INT, [P, INT]),
+    "sa_ctx_profile": (INT, [P, C.c_char_p, P, P]),
+    "sa_ctx_profile_reset": (INT, [P]),
+    "sa_ctx_mark": (INT, [P, INT]),
+    "sa_ctx_elapsed": (INT, [P, INT, INT, P]),
+    "sa_bsm_create": (INT, [P, I64, I64, I32, I32, D, P]),
+    "sa_bsm_create_blocked": (INT, [P, I32, P, P, I32, P, P, D, P]),
+    "sa_bsm_destroy": (INT, [P]),
+    "sa_bsm_info": (INT, [P, P, P, P, P, P, P]),
+    "sa_bsm_copy": (INT, [P, P]),
+    "sa_bsm_from_dense": (INT, [P, P, I64]),
+    "sa_bsm_to_dense": (INT, [P, P, I64]),
+    "sa_bsm_get_meta": (INT, [P, P, P]),
+    "sa_bsm_generate": (INT, [P, C.c_uint64, P]),
+    "sa_bsm_fill": (INT, [P, D]),
+    "sa_bsm_zero": (INT, [P]),
+    "sa_bsm_scale": (INT, [P, D]),
+    "sa_bsm_calc_norms": (INT, [P]),
+    "sa_bsm_recompress": (INT, [P]),
+    "sa_bsm_add": (INT, [P, P, D]),
+    "sa_bsm_add_transpose": (INT, [P, P]),
+    "sa_bsm_frobenius_norm": (INT, [P, P]),
+    "sa_bsm_dot": (INT, [P, P, D, P]),
+    "sa_matmult": (INT, [P, P, INT, P, INT, D, D, D, P]),
+    "sa_tasklist_get": (INT, [P, P, I64, P]),
+}
+
+
+class SAError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"strassen_amd error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"HIP library not built: {LIB_PATH} (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in ABI.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != SA_OK:
+        raise SAError(rc, lib().sa_last_error().decode())
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    _check(lib().sa_device_count(C.byref(n)))
+    return n.value
+
+
+def default_opts(**kw) -> Opts:
+    o = Opts()
+    lib().sa_opts_default(C.byref(o))
+    for k, v in kw.items():
+        if k == "tree":
+            arr = np.ascontiguousarray(np.concatenate(v) if isinstance(v, (list, tuple)) else v, dtype=np.uint8)
+            o._tree_keep = arr
+            o.tree = arr.ctypes.data
+        else:
+            setattr(o, k, v)
+    return o
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        self.h = P()
+        _check(lib().sa_ctx_create(device, C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().sa_ctx_destroy(self.h)
+            self.h = P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def synchronize(self):
+        _check(lib().sa_ctx_synchronize(self.h))
+
+    def set_profiling(self, on: bool):
+        _check(lib().sa_ctx_set_profiling(self.h, int(on)))
+
+    def profile(self, cls: str):
+        ms = D(0.0)
+        n = I64(0)
+        _check(lib().sa_ctx_profile(self.h, cls.encode(), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def profile_reset(self):
+        _check(lib().sa_ctx_profile_reset(self.h))
+
+    def mark(self, slot: int):
+        _check(lib().sa_ctx_mark(self.h, slot))
+
+    def elapsed(self, s0: int, s1: int) -> float:
+        ms = C.c_float(0.0)
+        _check(lib().sa_ctx_elapsed(self.h, s0, s1, C.byref(ms)))
+        return ms.value
+
+    def tasklist(self, cap=None):
+        n = I64(0)
+        _check(lib().sa_tasklist_get(self.h, None, 0, C.byref(n)))
+        cap = n.value if cap is None else cap
+        out = np.zeros((max(cap, 1), 3), dtype=np.int32)
+        _check(lib().sa_tasklist_get(self.h, _ptr(out), cap, C.byref(n)))
+        return out[: min(cap, n.value)]
+
+
+class BSM:
+    """Device block-sparse matrix (include/strassen_amd.h sa_bsm_*)."""
+
+    def __init__(self, ctx: Context, rsz, csz, thresh=0.0, rscr=None, cscr=None, _uniform=None):
+        self.ctx = ctx
+        self.h = P()
+        if _uniform is not None:
+            nrow, ncol, bsr, bsc = _uniform
+            _check(lib().sa_bsm_create(ctx.h, nrow, ncol, bsr, bsc, thresh, C.byref(self.h)))
+        else:
+            r = np.ascontiguousarray(rsz, dtype=np.int32)
+            c = np.ascontiguousarray(csz, dtype=np.int32)
+            rs = None if rscr is None else np.ascontiguousarray(rscr, dtype=np.int32)
+            cs = None if cscr is None else np.ascontiguousarray(cscr, dtype=np.int32)
+            _check(lib().sa_bsm_create_blocked(ctx.h, len(r), _ptr(r), None if rs is None else _ptr(rs),
+                                               len(c), _ptr(c), None if cs is None else _ptr(cs), thresh,
+                                               C.byref(self.h)))
+        self.nrow, self.ncol, self.nrb, self.ncb, _, _ = self.info(False)
+
+    @classmethod
+    def uniform(cls, ctx, nrow, ncol, bs_r, bs_c, thresh=0.0):
+        return cls(ctx, None, None, thresh, _uniform=(nrow, ncol, bs_r, bs_c))
+
+    def close(self):
+        if self.h:
+            lib().sa_bsm_destroy(self.h)
+            self.h = P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self, with_nalloc=True):
+        nr, nc, nalloc, ab = I64(0), I64(0), I64(0), I64(0)
+        rb, cb = I32(0), I32(0)
+        _check(lib().sa_bsm_info(self.h, C.byref(nr), C.byref(nc), C.byref(rb), C.byref(cb),
+                                 C.byref(nalloc) if with_nalloc else None, C.byref(ab)))
+        return nr.value, nc.value, rb.value, cb.value, nalloc.value, ab.value
+
+    def nalloc(self):
+        return self.info(True)[4]
+
+    def copy_from(self, src):
+        _check(lib().sa_bsm_copy(self.h, src.h))
+        return self
+
+    def from_dense(self, a):
+        a = np.asfortranarray(a, dtype=np.float64)
+        assert a.shape == (self.nrow, self.ncol), (a.shape, self.nrow, self.ncol)
+        _check(lib().sa_bsm_from_dense(self.h, _ptr(a), self.nrow))
+        return self
+
+    def to_dense(self):
+        out = np.zeros((self.nrow, self.ncol), dtype=np.float64, order="F")
+        _check(lib().sa_bsm_to_dense(self.h, _ptr(out), self.nrow))
+        return out
+
+    def meta(self):
+        nb = self.nrb * self.ncb
+        pres = np.zeros(max(nb, 1), dtype=np.uint8)
+        norms = np.zeros(max(nb, 1), dtype=np.float64)
+        _check(lib().sa_bsm_get_meta(self.h, _ptr(pres), _ptr(norms)))
+        return (pres[:nb].reshape(self.nrb, self.ncb, order="F").astype(bool),
+                norms[:nb].reshape(self.nrb, self.ncb, order="F"))
+
+    def generate(self, seed, block_scale):
+        s = np.asfortranarray(block_scale, dtype=np.float64)
+        assert s.shape == (self.nrb, self.ncb)
+        flat = np.ascontiguousarray(s.ravel(order="F"))
+        _check(lib().sa_bsm_generate(self.h, seed, _ptr(flat)))
+        return self
+
+    def fill(self, v):
+        _check(lib().sa_bsm_fill(self.h, v))
+
+    def zero(self):
+        _check(lib().sa_bsm_zero(self.h))
+
+    def scale(self, s):
+        _check(lib().sa_bsm_scale(self.h, s))
+
+    def calc_norms(self):
+        _check(lib().sa_bsm_calc_norms(self.h))
+
+    def recompress(self):
+        _check(lib().sa_bsm_recompress(self.h))
+
+    def add(self, rhs, sign=1.0):
+        _check(lib().sa_bsm_add(self.h, rhs.h, sign))
+
+    def add_transpose(self, rhs):
+        _check(lib().sa_bsm_add_transpose(self.h, rhs.h))
+
+    def frobenius_norm(self):
+        out = D(0.0)
+        _check(lib().sa_bsm_frobenius_norm(self.h, C.byref(out)))
+        return out.value
+
+    def dot(self, other, thresh=0.0):
+        out = D(0.0)
+        _check(lib().sa_bsm_dot(self.h, other.h, thresh, C.byref(out)))
+        return out.value
+
+
+def matmult(Cm: BSM, A: BSM, transA, B: BSM, transB, thresh, alpha=1.0, beta=0.0, opts: Opts | None = None):
+    _check(lib().sa_matmult(Cm.h, A.h, int(transA), B.h, int(transB), thresh, alpha, beta,
+                            None if opts is None else C.byref(opts)))

@@ -1,0 +1,304 @@
+// sa_context.hip -- context, error state, device buffers, profiling and the
+// device prefix scan shared by the table builders.
+#include <cstdarg>
+#include <cstring>
+
+#include "sa_internal.hpp"
+
+namespace sa {
+
+static thread_local char g_err[1024] = "";
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+void DevBuf::reserve(size_t n) {
+  if (n <= bytes) return;
+  if (p) SA_CHECK_HIP(hipFree(p));
+  p = nullptr;
+  bytes = 0;
+  size_t want = n < 4096 ? 4096 : n;
+  SA_CHECK_HIP(hipMalloc(&p, want));
+  bytes = want;
+}
+
+void DevBuf::reserve_keep(size_t n, hipStream_t s) {
+  if (n <= bytes) return;
+  size_t want = bytes + bytes / 2;
+  if (want < n) want = n;
+  if (want < 4096) want = 4096;
+  void* q = nullptr;
+  SA_CHECK_HIP(hipMalloc(&q, want));
+  if (p) {
+    SA_CHECK_HIP(hipMemcpyAsync(q, p, bytes, hipMemcpyDeviceToDevice, s));
+    SA_CHECK_HIP(hipStreamSynchronize(s));
+    SA_CHECK_HIP(hipFree(p));
+  }
+  p = q;
+  bytes = want;
+}
+
+void DevBuf::release() {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  bytes = 0;
+}
+
+void* pinned(sa_context* c, size_t bytes) {
+  if (bytes > c->pinned_bytes) {
+    if (c->pinned) SA_CHECK_HIP(hipHostFree(c->pinned));
+    c->pinned = nullptr;
+    c->pinned_bytes = 0;
+    size_t want = bytes < 65536 ? 65536 : bytes;
+    SA_CHECK_HIP(hipHostMalloc(&c->pinned, want, hipHostMallocDefault));
+    c->pinned_bytes = want;
+  }
+  return c->pinned;
+}
+
+void prof_begin(sa_context* c, const char* cls, hipEvent_t* ev0) {
+  (void)cls;
+  *ev0 = nullptr;
+  if (!c->profiling) return;
+  SA_CHECK_HIP(hipEventCreate(ev0));
+  SA_CHECK_HIP(hipEventRecord(*ev0, c->stream));
+}
+
+void prof_end(sa_context* c, const char* cls, hipEvent_t ev0) {
+  if (!c->profiling || !ev0) return;
+  hipEvent_t ev1;
+  if (hipEventCreate(&ev1) != hipSuccess) return;
+  if (hipEventRecord(ev1, c->stream) != hipSuccess) return;
+  c->pending.push_back({std::string(cls), {ev0, ev1}});
+}
+
+void prof_collect(sa_context* c) {
+  for (auto& p : c->pending) {
+    float ms = 0.f;
+    SA_CHECK_HIP(hipEventSynchronize(p.second.second));
+    SA_CHECK_HIP(hipEventElapsedTime(&ms, p.second.first, p.second.second));
+    auto& acc = c->prof[p.first];
+    acc.ms += ms;
+    acc.launches += 1;
+    (void)hipEventDestroy(p.second.first);
+    (void)hipEventDestroy(p.second.second);
+  }
+  c->pending.clear();
+}
+
+// ------------------------------------------------------------------- scan --
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+__global__ void __launch_bounds__(kScanThreads) k_scan_tiles(int64_t* __restrict__ d, int64_t n,
+                                                              int64_t* __restrict__ tile_sums) {
+  __shared__ int64_t wsum[kScanThreads / 64];
+  const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+  int64_t v[kScanItems];
+  int64_t run = 0;
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    int64_t t = (base + i < n) ? d[base + i] : 0;
+    v[i] = run;
+    run += t;
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int64_t x = run;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    int64_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wsum[w] = x;
+  __syncthreads();
+  int64_t wpre = 0;
+  for (int i = 0; i < w; ++i) wpre += wsum[i];
+  const int64_t excl = wpre + x - run;
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i)
+    if (base + i < n) d[base + i] = v[i] + excl;
+  if (threadIdx.x == kScanThreads - 1) tile_sums[blockIdx.x] = wpre + x;
+}
+
+__global__ void k_scan_add(int64_t* __restrict__ d, int64_t n, const int64_t* __restrict__ tile_off) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[i] += tile_off[i / kScanTile];
+}
+
+static void scan_rec(sa_context* c, int64_t* d, int64_t n, int64_t* ws, int64_t* total_slot) {
+  const int64_t ntiles = ceil_div(n, kScanTile);
+  int64_t* part = ws;
+  hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)ntiles), dim3(kScanThreads), 0, c->stream, d, n, part);
+  if (ntiles == 1) {
+    SA_CHECK_HIP(hipMemcpyAsync(total_slot, part, sizeof(int64_t), hipMemcpyDeviceToDevice, c->stream));
+    return;
+  }
+  scan_rec(c, part, ntiles, ws + ntiles, total_slot);
+  hipLaunchKernelGGL(k_scan_add, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, c->stream, d, n,
+                     (const int64_t*)part);
+}
+
+void scan_exclusive_i64_async(sa_context* c, int64_t* d, int64_t n, int64_t* total_dev) {
+  int64_t need = 2, m = n;
+  while (m > 1) {
+    m = ceil_div(m, kScanTile);
+    need += m;
+  }
+  c->ws_scan.reserve(sizeof(int64_t) * (size_t)need);
+  int64_t* ws = c->ws_scan.as<int64_t>();
+  if (n == 0) {
+    fill_i64(c, total_dev, 1, 0);
+  } else {
+    scan_rec(c, d, n, ws, total_dev);
+  }
+  SA_CHECK_HIP(hipGetLastError());
+}
+
+int64_t scan_exclusive_i64(sa_context* c, int64_t* d, int64_t n, bool read_total) {
+  // workspace: partial sums of every level + 1 total slot
+  int64_t need = 2, m = n;
+  while (m > 1) {
+    m = ceil_div(m, kScanTile);
+    need += m;
+  }
+  c->ws_scan.reserve(sizeof(int64_t) * (size_t)need);
+  int64_t* ws = c->ws_scan.as<int64_t>();
+  int64_t* total_slot = ws + need - 1;
+  if (n == 0) {
+    fill_i64(c, total_slot, 1, 0);
+  } else {
+    scan_rec(c, d, n, ws, total_slot);
+  }
+  SA_CHECK_HIP(hipGetLastError());
+  if (!read_total) return 0;
+  return read_i64(c, total_slot);
+}
+
+int64_t read_i64(sa_context* c, const int64_t* d) {
+  int64_t* h = static_cast<int64_t*>(pinned(c, sizeof(int64_t)));
+  SA_CHECK_HIP(hipMemcpyAsync(h, d, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+  SA_CHECK_HIP(hipStreamSynchronize(c->stream));
+  return *h;
+}
+
+}  // namespace sa
+
+// ===================================================================== ABI ==
+using namespace sa;
+
+extern "C" {
+
+void sa_opts_default(sa_opts_t* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->mode = SA_MODE_FP64;
+  o->slices = 4;
+  o->slice_type = SA_SLICE_FP16;
+  o->tile = -1;
+  o->scheduler = SA_SCHED_HEURISTIC;
+  o->cutoff = 0;
+  o->tree = nullptr;
+}
+
+const char* sa_last_error(void) { return g_err; }
+
+int sa_device_count(int* n) {
+  return guard([&] {
+    SA_REQUIRE(n, "null output");
+    SA_CHECK_HIP(hipGetDeviceCount(n));
+  });
+}
+
+int sa_ctx_create(int device, sa_ctx_t* out) {
+  return guard([&] {
+    SA_REQUIRE(out, "null output");
+    int n = 0;
+    SA_CHECK_HIP(hipGetDeviceCount(&n));
+    SA_REQUIRE(device >= 0 && device < n, "device %d out of range (%d devices)", device, n);
+    SA_CHECK_HIP(hipSetDevice(device));
+    auto* c = new sa_context();
+    c->device = device;
+    SA_CHECK_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    hipDeviceProp_t prop;
+    SA_CHECK_HIP(hipGetDeviceProperties(&prop, device));
+    c->num_cus = prop.multiProcessorCount;
+    for (auto& e : c->marks) SA_CHECK_HIP(hipEventCreate(&e));
+    *out = c;
+  });
+}
+
+int sa_ctx_destroy(sa_ctx_t c) {
+  return guard([&] {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (auto& p : c->pending) {
+      (void)hipEventDestroy(p.second.first);
+      (void)hipEventDestroy(p.second.second);
+    }
+    for (auto& e : c->marks) (void)hipEventDestroy(e);
+    c->ws_scan.release(); c->ws_tmp0.release(); c->ws_tmp1.release(); c->ws_tmp2.release();
+    c->ws_stage.release(); c->ws_gemm_items.release(); c->ws_gemm_items2.release();
+    for (auto& kv : c->bufs) kv.second.release();
+    if (c->pinned) (void)hipHostFree(c->pinned);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+  });
+}
+
+int sa_ctx_synchronize(sa_ctx_t c) {
+  return guard([&] {
+    SA_REQUIRE(c, "null context");
+    SA_CHECK_HIP(hipStreamSynchronize(c->stream));
+    prof_collect(c);
+  });
+}
+
+int sa_ctx_set_profiling(sa_ctx_t c, int on) {
+  return guard([&] {
+    SA_REQUIRE(c, "null context");
+    c->profiling = on != 0;
+  });
+}
+
+int sa_ctx_profile(sa_ctx_t c, const char* cls, double* ms, int64_t* launches) {
+  return guard([&] {
+    SA_REQUIRE(c && cls, "null argument");
+    SA_CHECK_HIP(hipStreamSynchronize(c->stream));
+    prof_collect(c);
+    auto it = c->prof.find(cls);
+    if (ms) *ms = it == c->prof.end() ? 0.0 : it->second.ms;
+    if (launches) *launches = it == c->prof.end() ? 0 : it->second.launches;
+  });
+}
+
+int sa_ctx_profile_reset(sa_ctx_t c) {
+  return guard([&] {
+    SA_REQUIRE(c, "null context");
+    SA_CHECK_HIP(hipStreamSynchronize(c->stream));
+    prof_collect(c);
+    c->prof.clear();
+  });
+}
+
+int sa_ctx_mark(sa_ctx_t c, int slot) {
+  return guard([&] {
+    SA_REQUIRE(c && slot >= 0 && slot < 16, "bad mark slot");
+    SA_CHECK_HIP(hipEventRecord(c->marks[slot], c->stream));
+  });
+}
+
+int sa_ctx_elapsed(sa_ctx_t c, int s0, int s1, float* ms) {
+  return guard([&] {
+    SA_REQUIRE(c && ms && s0 >= 0 && s0 < 16 && s1 >= 0 && s1 < 16, "bad mark slot");
+    SA_CHECK_HIP(hipEventSynchronize(c->marks[s1]));
+    SA_CHECK_HIP(hipEventElapsedTime(ms, c->marks[s0], c->marks[s1]));
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,203 @@
+// sa_gemm.hip -- grouped FP64 block GEMM on CDNA4 matrix cores.
+//
+// One launch runs a whole batch of output blocks of uneven sizes.  Each work
+// item is one output tile (TM x TN) of one output block; its workgroup keeps the
+// tile's accumulators in registers across every product of the block's
+// segment (C-stationary), stages op(A) / op(B) k-slices of KT through LDS with
+// loads coalesced along each stored block's contiguous dimension, and issues
+// v_mfma_f64_16x16x4_f64 per 16x16 sub-tile.  C is written once at the end:
+//   C = alpha * sum_p op(A_p) op(B_p) + (acc ? C : 0).
+//
+// MFMA operand layout (wave64, 16x16x4 f64):
+//   A: lane l holds A[l%16][l/16]     (row i, k index)
+//   B: lane l holds B[l/16][l%16]     (k index, column j)
+//   D: lane l, element v holds D[4*(l/16)+v][l%16]
+// Work items are remapped so that consecutive items (which share A/B blocks)
+// land on the same XCD (hardware dispatches workgroup i to XCD i % 8).
+#include <algorithm>
+
+#include "sa_internal.hpp"
+
+namespace sa {
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+__device__ inline int64_t xcd_remap(int64_t bid, int64_t n) {
+  const int64_t q = n / 8, r = n % 8;
+  const int64_t x = bid % 8, t = bid / 8;
+  return x * q + (x < r ? x : r) + t;
+}
+
+template <int WM, int WN, int FM, int FN, int KT, bool TA, bool TB>
+__global__ void __launch_bounds__(64 * WM * WN)
+    k_gemm_fp64(const GemmOut* __restrict__ outs, const GemmProd* __restrict__ prods,
+                const GemmItem* __restrict__ items, int64_t nitems, double alpha) {
+  constexpr int NT = 64 * WM * WN;
+  constexpr int TM = 16 * FM * WM;
+  constexpr int TN = 16 * FN * WN;
+  constexpr int LDA = TM + 2;
+  constexpr int LDB = TN + 2;
+  static_assert((TM * KT) % NT == 0 && (TN * KT) % NT == 0, "tile/threads mismatch");
+  static_assert(KT % 4 == 0, "KT must be a multiple of the MFMA k (4)");
+  __shared__ double As[KT][LDA];
+  __shared__ double Bs[KT][LDB];
+
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int wm = wid % WM, wn = wid / WM;
+  const int64_t it = xcd_remap(blockIdx.x, nitems);
+  const GemmItem item = items ? items[it] : GemmItem{(int32_t)it, 0, 0};
+  const GemmOut o = outs[item.out];
+  const int m = o.m, n = o.n;
+  const int m0 = item.tm * TM, n0 = item.tn * TN;
+
+  dbl4 acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
+
+  for (int p = o.p0; p < o.p1; ++p) {
+    const GemmProd pr = prods[p];
+    const int k = pr.k;
+    for (int k0 = 0; k0 < k; k0 += KT) {
+#pragma unroll
+      for (int r = 0; r < (TM * KT) / NT; ++r) {
+        const int e = tid + r * NT;
+        const int i = TA ? e / KT : e % TM;
+        const int kk = TA ? e % KT : e / TM;
+        const int gi = m0 + i, gk = k0 + kk;
+        double v = 0.0;
+        if (gi < m && gk < k) v = TA ? pr.a[gk + (int64_t)gi * k] : pr.a[gi + (int64_t)gk * m];
+        As[kk][i] = v;
+      }
+#pragma unroll
+      for (int r = 0; r < (TN * KT) / NT; ++r) {
+        const int e = tid + r * NT;
+        const int j = TB ? e % TN : e / KT;
+        const int kk = TB ? e / TN : e % KT;
+        const int gj = n0 + j, gk = k0 + kk;
+        double v = 0.0;
+        if (gj < n && gk < k) v = TB ? pr.b[gj + (int64_t)gk * n] : pr.b[gk + (int64_t)gj * k];
+        Bs[kk][j] = v;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int ks = 0; ks < KT / 4; ++ks) {
+        const int kr = ks * 4 + (lane >> 4);
+        double af[FM], bf[FN];
+#pragma unroll
+        for (int i = 0; i < FM; ++i) af[i] = As[kr][(wm * FM + i) * 16 + (lane & 15)];
+#pragma unroll
+        for (int j = 0; j < FN; ++j) bf[j] = Bs[kr][(wn * FN + j) * 16 + (lane & 15)];
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+      }
+      __syncthreads();
+    }
+  }
+
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const int col = n0 + (wn * FN + j) * 16 + (lane & 15);
+      const int rowb = m0 + (wm * FM + i) * 16 + 4 * (lane >> 4);
+      if (col >= n) continue;
+      double* cp = o.c + (int64_t)col * m;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = rowb + v;
+        if (row < m) {
+          double x = alpha * acc[i][j][v];
+          if (o.acc) x = x + cp[row];
+          cp[row] = x;
+        }
+      }
+    }
+}
+
+// ----------------------------------------------------------- work items --
+__global__ void k_count_tiles(const GemmOut* __restrict__ outs, int64_t nouts, int TM, int TN,
+                              int64_t* __restrict__ cnt) {
+  const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= nouts) return;
+  cnt[o] = ceil_div(outs[o].m, TM) * ceil_div(outs[o].n, TN);
+}
+
+__global__ void k_emit_items(const GemmOut* __restrict__ outs, int64_t nouts, int TM, int TN,
+                             const int64_t* __restrict__ start, GemmItem* __restrict__ items) {
+  const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= nouts) return;
+  const int tm = (int)ceil_div(outs[o].m, TM), tn = (int)ceil_div(outs[o].n, TN);
+  int64_t s = start[o];
+  for (int b = 0; b < tn; ++b)
+    for (int a = 0; a < tm; ++a) items[s++] = GemmItem{(int32_t)o, (int16_t)a, (int16_t)b};
+}
+
+// ------------------------------------------------------------- configs --
+using GemmFn = void (*)(const GemmOut*, const GemmProd*, const GemmItem*, int64_t, double);
+
+struct Cfg {
+  const char* name;
+  int tm, tn, threads;
+  GemmFn fn[4];  // [ta*2+tb]
+};
+
+#define SA_CFG(NAME, WM, WN, FM, FN, KT)                                                           \
+  Cfg {                                                                                           \
+    NAME, 16 * FM * WM, 16 * FN * WN, 64 * WM * WN,                                               \
+        {k_gemm_fp64<WM, WN, FM, FN, KT, false, false>, k_gemm_fp64<WM, WN, FM, FN, KT, false, true>, \
+         k_gemm_fp64<WM, WN, FM, FN, KT, true, false>, k_gemm_fp64<WM, WN, FM, FN, KT, true, true>} \
+  }
+
+static const Cfg kCfgs[] = {
+    SA_CFG("64x64_w4_k16", 2, 2, 2, 2, 16),   // 0: uniform 64 blocks
+    SA_CFG("128x128_w4_k16", 2, 2, 4, 4, 16), // 1: uniform 128 blocks
+    SA_CFG("32x32_w1_k16", 1, 1, 2, 2, 16),   // 2: small blocks, one wave
+    SA_CFG("48x48_w1_k8", 1, 1, 3, 3, 8),     // 3: atom blocks <= 48
+    SA_CFG("16x16_w1_k8", 1, 1, 1, 1, 8),     // 4: tiny blocks
+    SA_CFG("32x32_w4_k16", 2, 2, 1, 1, 16),   // 5: 32 tile, four waves
+    SA_CFG("64x64_w4_k8", 2, 2, 2, 2, 8),     // 6
+    SA_CFG("128x64_w4_k16", 2, 2, 4, 2, 16),  // 7
+};
+static constexpr int kNumCfgs = (int)(sizeof(kCfgs) / sizeof(kCfgs[0]));
+
+int gemm_fp64_num_configs() { return kNumCfgs; }
+const char* gemm_fp64_config_name(int cfg) { return (cfg >= 0 && cfg < kNumCfgs) ? kCfgs[cfg].name : "?"; }
+
+static int heuristic_cfg(const GemmBatch& b) {
+  const int mx = std::max(b.max_m, b.max_n);
+  if (mx > 96) return 1;
+  if (mx > 48) return 0;
+  if (mx > 32) return 3;
+  if (mx > 16) return 2;
+  return 4;
+}
+
+void gemm_fp64_run(sa_context* c, const GemmBatch& b, int cfg) {
+  if (b.nouts == 0) return;
+  if (cfg < 0 || cfg >= kNumCfgs) cfg = heuristic_cfg(b);
+  const Cfg& k = kCfgs[cfg];
+  const GemmItem* items = nullptr;
+  int64_t nitems = b.nouts;
+  if (b.max_m > k.tm || b.max_n > k.tn) {
+    c->ws_gemm_items.reserve(sizeof(int64_t) * (size_t)(b.nouts + 1));
+    int64_t* cnt = c->ws_gemm_items.as<int64_t>();
+    hipLaunchKernelGGL(k_count_tiles, dim3((unsigned)ceil_div(b.nouts, 256)), dim3(256), 0, c->stream, b.outs, b.nouts,
+                       k.tm, k.tn, cnt);
+    nitems = scan_exclusive_i64(c, cnt, b.nouts);
+    c->ws_gemm_items2.reserve(sizeof(GemmItem) * (size_t)(nitems + 1));
+    hipLaunchKernelGGL(k_emit_items, dim3((unsigned)ceil_div(b.nouts, 256)), dim3(256), 0, c->stream, b.outs,
+                       b.nouts, k.tm, k.tn, (const int64_t*)cnt, c->ws_gemm_items2.as<GemmItem>());
+    items = c->ws_gemm_items2.as<GemmItem>();
+  }
+  SA_REQUIRE(nitems < (int64_t)1 << 31, "too many GEMM work items");
+  ProfScope ps(c, "gemm_fp64");
+  hipLaunchKernelGGL(k.fn[b.ta * 2 + b.tb], dim3((unsigned)nitems), dim3(k.threads), 0, c->stream, b.outs, b.prods,
+                     items, nitems, b.alpha);
+  SA_CHECK_HIP(hipGetLastError());
+}
+
+}  // namespace sa

@@ -1,0 +1,209 @@
+// sa_internal.hpp -- internal types of the MI355X block-sparse multiply library.
+//
+// Device layout of a block-sparse matrix (HBM):
+//   arena   : one device allocation per matrix holding every present block,
+//             each block column-major (ld = its row count), blocks packed by a
+//             bump pointer in units of doubles;
+//   off[b]  : int64 element offset of block b in the arena, -1 = empty block;
+//   norm[b] : cached Frobenius norm of block b (FP64);
+//   b = rb + cb*nrb (+ l*nrb*ncb for a batch slice l), the reference's
+//   column-major block grid (blocksparsematrix.h:71-72).
+// Block geometry (sizes, screening sizes, row/column starts) lives in small
+// device arrays next to host copies.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/strassen_amd.h"
+
+namespace sa {
+
+// ------------------------------------------------------------------ errors --
+void set_error(const char* fmt, ...);
+
+struct Error {
+  int code;
+};
+
+#define SA_CHECK_HIP(expr)                                                            \
+  do {                                                                                \
+    hipError_t _e = (expr);                                                           \
+    if (_e != hipSuccess) {                                                           \
+      ::sa::set_error("%s:%d: %s -> %s", __FILE__, __LINE__, #expr, hipGetErrorString(_e)); \
+      throw ::sa::Error{_e == hipErrorOutOfMemory ? SA_ERR_OOM : SA_ERR_HIP};         \
+    }                                                                                 \
+  } while (0)
+
+#define SA_REQUIRE(cond, ...)          \
+  do {                                 \
+    if (!(cond)) {                     \
+      ::sa::set_error(__VA_ARGS__);    \
+      throw ::sa::Error{SA_ERR_INVALID}; \
+    }                                  \
+  } while (0)
+
+#define SA_UNSUPPORTED(...)                 \
+  do {                                      \
+    ::sa::set_error(__VA_ARGS__);           \
+    throw ::sa::Error{SA_ERR_UNSUPPORTED};  \
+  } while (0)
+
+// Wrap an ABI body: exceptions become status codes.
+template <class F>
+int guard(F&& f) {
+  try {
+    f();
+    return SA_OK;
+  } catch (const Error& e) {
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_error("host allocation failed");
+    return SA_ERR_OOM;
+  }
+}
+
+// -------------------------------------------------------------- device mem --
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  void reserve(size_t n);            // grow-only, contents discarded
+  void reserve_keep(size_t n, hipStream_t s);  // grow-only, contents kept
+  void release();
+  template <class T>
+  T* as() const { return static_cast<T*>(p); }
+};
+
+__host__ __device__ inline int64_t ceil_div(int64_t a, int64_t b) { return a == 0 ? 0 : 1 + (a - 1) / b; }
+
+// --------------------------------------------------------------- profiling --
+struct ProfAcc {
+  double ms = 0.0;
+  int64_t launches = 0;
+};
+
+}  // namespace sa
+
+struct sa_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int num_cus = 256;
+  // scratch workspaces (grow-only)
+  sa::DevBuf ws_scan, ws_tmp0, ws_tmp1, ws_tmp2, ws_stage, ws_gemm_items, ws_gemm_items2;
+  // named grow-only buffers of the multiply pipelines (task lists, records)
+  std::map<std::string, sa::DevBuf> bufs;
+  // last task list (canonical order): per C block count ("mm_cnt"), exclusive
+  // start ("mm_seg") and the kb list ("mm_kb")
+  int64_t last_ntasks = 0, last_nc = 0;
+  int32_t last_nib = 0;
+  sa::DevBuf& buf(const char* name) { return bufs[name]; }
+  // profiling
+  bool profiling = false;
+  std::map<std::string, sa::ProfAcc> prof;
+  std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
+  hipEvent_t marks[16] = {};
+  // pinned host staging for small readbacks
+  void* pinned = nullptr;
+  size_t pinned_bytes = 0;
+};
+
+struct sa_bsm {
+  sa_context* ctx = nullptr;
+  int32_t nrb = 0, ncb = 0;
+  int64_t nrow = 0, ncol = 0;
+  double thresh = 0.0;
+  bool uniform = false;
+  int32_t bs_r = 0, bs_c = 0;
+  std::vector<int32_t> rsz, csz, rscr, cscr;
+  std::vector<int64_t> roff, coff;  // start row of block row rb, start col of block col cb
+  // device geometry
+  int32_t *d_rsz = nullptr, *d_csz = nullptr, *d_rscr = nullptr, *d_cscr = nullptr;
+  int64_t *d_roff = nullptr, *d_coff = nullptr;
+  // device block table
+  int64_t* d_off = nullptr;
+  double* d_norm = nullptr;
+  // arena
+  sa::DevBuf arena;
+  int64_t used = 0;  // doubles in use (bump pointer)
+  int64_t nb() const { return (int64_t)nrb * ncb; }
+  double* data() const { return arena.as<double>(); }
+};
+
+namespace sa {
+
+// context helpers
+void* pinned(sa_context* c, size_t bytes);
+void prof_begin(sa_context* c, const char* cls, hipEvent_t* ev0);
+void prof_end(sa_context* c, const char* cls, hipEvent_t ev0);
+void prof_collect(sa_context* c);  // after a synchronize
+
+struct ProfScope {
+  sa_context* c;
+  const char* cls;
+  hipEvent_t ev0 = nullptr;
+  ProfScope(sa_context* c_, const char* cls_) : c(c_), cls(cls_) { prof_begin(c, cls, &ev0); }
+  ~ProfScope() { prof_end(c, cls, ev0); }
+};
+
+// exclusive scan of n int64 values in place (device), returns the total (sync)
+int64_t scan_exclusive_i64(sa_context* c, int64_t* d, int64_t n, bool read_total = true);
+// same, total written to a device slot, no synchronisation
+void scan_exclusive_i64_async(sa_context* c, int64_t* d, int64_t n, int64_t* total_dev);
+// read one int64 from device (sync)
+int64_t read_i64(sa_context* c, const int64_t* d);
+
+// small fill kernels (used instead of memset so the work can sit in a HIP graph)
+void fill_i64(sa_context* c, int64_t* p, int64_t n, int64_t v);
+void fill_f64(sa_context* c, double* p, int64_t n, double v);
+
+// arena management
+void bsm_ensure_capacity(sa_bsm* m, int64_t elems);
+void bsm_upload_geometry(sa_bsm* m);
+void bsm_alloc_tables(sa_bsm* m);
+
+// norm kernel launcher: recompute norms of every present block (or all if mask==nullptr)
+void launch_block_norms(sa_bsm* m);
+
+// ------------------------------------------------------------ grouped GEMM --
+// One output block (column-major, ld = m) and its segment [p0, p1) of products
+// C = alpha * sum_p op(A_p) op(B_p) + (acc ? C : 0).
+struct GemmOut {
+  double* c;
+  int32_t m, n;
+  int32_t p0, p1;
+  int32_t acc;
+  int32_t pad;
+};
+// op(A) is m x k, op(B) is k x n; stored blocks are column-major with
+// ld = rows of the stored block (A: m or k, B: k or n).
+struct GemmProd {
+  const double* a;
+  const double* b;
+  int32_t k;
+  int32_t pad;
+};
+struct GemmItem {
+  int32_t out;
+  int16_t tm, tn;
+};
+struct GemmBatch {
+  const GemmOut* outs = nullptr;  // device
+  int64_t nouts = 0;
+  const GemmProd* prods = nullptr;  // device
+  int64_t nprods = 0;
+  int ta = 0, tb = 0;
+  double alpha = 1.0;
+  int32_t max_m = 0, max_n = 0, max_k = 0;
+  double flops = 0.0;  // algorithmic flops (for autotuning statistics)
+};
+// Tile configurations of the FP64 MFMA kernel.
+int gemm_fp64_num_configs();
+const char* gemm_fp64_config_name(int cfg);
+// Run a batch with tile config cfg (-1: autotuned choice for this shape class).
+void gemm_fp64_run(sa_context* c, const GemmBatch& b, int cfg);
+
+}  // namespace sa

@@ -1,0 +1,342 @@
+// sa_matmult.hip -- norm-screened block GEMM C = alpha op(A) op(B) + beta C
+// (blocksparsematrix.hpp:292-414) on the device.
+//
+// Pipeline (all on the context stream):
+//  1. beta pre-pass (zero / scale), as the reference;
+//  2. screening, pass 1: one wave per C block (ib, jb) walks kb in chunks of
+//     64 lanes, tests  A(ib,kb), B(kb,jb) present  and
+//     |A||B| >= thresh * s_i * s_j * s_k,  counts survivors with a wavefront
+//     ballot;
+//  3. device exclusive scans: task segments, new-C-block arena offsets, the
+//     compacted list of non-empty outputs (one host readback of 3 totals);
+//  4. screening, pass 2: the same walk writes each survivor kb at its
+//     ballot-prefix position -- the canonical task list (C blocks column-major,
+//     kb ascending), bit-identical to the oracle's;
+//  5. record build: one GemmOut per non-empty C block, one GemmProd per task;
+//  6. one grouped FP64 MFMA launch over all output tiles.
+// A C block created here starts with a zero cached norm, like the reference's
+// Mat(ni, nj, alloc) at blocksparsematrix.hpp:398-404.
+#include <cstring>
+
+#include "sa_internal.hpp"
+
+namespace sa {
+
+struct ScreenArgs {
+  const int64_t* aoff;
+  const double* anorm;
+  const int64_t* boff;
+  const double* bnorm;
+  const int32_t* iscr;  // screening sizes of op(A) rows
+  const int32_t* jscr;  // screening sizes of op(B) columns
+  const int32_t* kscr;  // screening sizes of the inner dimension
+  int32_t a_nrb, b_nrb;
+  int32_t nib, njb, nkb;
+  int32_t ta, tb;
+  double thresh;
+  // optional Strassen coverage: tree bits of levels [0, nlev) at lvl_off[l]
+  const uint8_t* tree;
+  const int64_t* lvl_off;
+  int32_t nlev;
+};
+
+__device__ inline bool covered_by_strassen(const ScreenArgs& s, int ib, int jb, int kb) {
+  for (int l = s.nlev - 1; l >= 0; --l) {
+    const int sh = l + 1;
+    const int64_t ns = ((int64_t)s.nib + (1 << sh) - 1) >> sh;
+    const int64_t idx = (int64_t)(ib >> sh) + (int64_t)(jb >> sh) * ns + (int64_t)(kb >> sh) * ns * ns;
+    if (s.tree[s.lvl_off[l] + idx]) return true;
+  }
+  return false;
+}
+
+// PASS 1: cnt[c] = number of surviving kb.  PASS 2: write kb at seg[c] + rank.
+template <int PASS>
+__global__ void __launch_bounds__(256) k_screen(ScreenArgs s, int64_t* __restrict__ cnt_or_seg,
+                                                int32_t* __restrict__ kb_out) {
+  const int lane = threadIdx.x & 63;
+  const int64_t c = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int64_t nc = (int64_t)s.nib * s.njb;
+  if (c >= nc) return;
+  const int ib = (int)(c % s.nib), jb = (int)(c / s.nib);
+  const int64_t sij = (int64_t)s.iscr[ib] * (int64_t)s.jscr[jb];
+  int64_t pos = PASS == 2 ? cnt_or_seg[c] : 0;
+  for (int k0 = 0; k0 < s.nkb; k0 += 64) {
+    const int kb = k0 + lane;
+    bool ok = false;
+    if (kb < s.nkb) {
+      const int64_t ai = s.ta ? (int64_t)kb + (int64_t)ib * s.a_nrb : (int64_t)ib + (int64_t)kb * s.a_nrb;
+      const int64_t bi = s.tb ? (int64_t)jb + (int64_t)kb * s.b_nrb : (int64_t)kb + (int64_t)jb * s.b_nrb;
+      if (s.aoff[ai] >= 0 && s.boff[bi] >= 0) {
+        const double est = s.anorm[ai] * s.bnorm[bi];
+        const double thr = s.thresh * (double)(sij * (int64_t)s.kscr[kb]);
+        ok = est >= thr;
+        if (ok && s.nlev > 0) ok = !covered_by_strassen(s, ib, jb, kb);
+      }
+    }
+    const uint64_t mask = __ballot(ok);
+    if (PASS == 2 && ok) {
+      const uint64_t below = mask & ((1ull << lane) - 1ull);
+      kb_out[pos + __popcll(below)] = kb;
+    }
+    pos += __popcll(mask);
+  }
+  if (PASS == 1 && lane == 0) cnt_or_seg[c] = pos;
+}
+
+struct MMTables {
+  const int64_t* cnt;
+  const int64_t* seg;
+  const int64_t* need_scan;  // exclusive scan of new-block sizes
+  const int64_t* out_scan;   // exclusive scan of (cnt > 0)
+  const int32_t* kb;
+  int64_t* coff;
+  double* cnorm;
+  double* carena;
+  const int32_t* crsz;
+  const int32_t* ccsz;
+  int32_t nib, njb;
+  int64_t cbase;  // C.used before this multiply
+  const int64_t* aoff;
+  const double* aarena;
+  int32_t a_nrb;
+  const int64_t* boff;
+  const double* barena;
+  int32_t b_nrb;
+  const int32_t* ksz;  // actual inner block sizes
+  int32_t ta, tb;
+};
+
+// one thread per C block: assign new C blocks, emit the output record and the
+// product records of its segment
+__global__ void k_build_records(MMTables t, GemmOut* __restrict__ outs, GemmProd* __restrict__ prods) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nc = (int64_t)t.nib * t.njb;
+  if (c >= nc) return;
+  const int64_t cnt = t.cnt[c];
+  if (cnt == 0) return;
+  const int ib = (int)(c % t.nib), jb = (int)(c / t.nib);
+  int64_t off = t.coff[c];
+  const bool fresh = off < 0;
+  if (fresh) {
+    off = t.cbase + t.need_scan[c];
+    t.coff[c] = off;
+    t.cnorm[c] = 0.0;
+  }
+  const int64_t seg = t.seg[c];
+  GemmOut o;
+  o.c = t.carena + off;
+  o.m = t.crsz[ib];
+  o.n = t.ccsz[jb];
+  o.p0 = (int32_t)seg;
+  o.p1 = (int32_t)(seg + cnt);
+  o.acc = fresh ? 0 : 1;
+  o.pad = 0;
+  outs[t.out_scan[c]] = o;
+  for (int64_t p = seg; p < seg + cnt; ++p) {
+    const int kb = t.kb[p];
+    const int64_t ai = t.ta ? (int64_t)kb + (int64_t)ib * t.a_nrb : (int64_t)ib + (int64_t)kb * t.a_nrb;
+    const int64_t bi = t.tb ? (int64_t)jb + (int64_t)kb * t.b_nrb : (int64_t)kb + (int64_t)jb * t.b_nrb;
+    GemmProd pr;
+    pr.a = t.aarena + t.aoff[ai];
+    pr.b = t.barena + t.boff[bi];
+    pr.k = t.ksz[kb];
+    pr.pad = 0;
+    prods[p] = pr;
+  }
+}
+
+__global__ void k_need_and_flags(const int64_t* __restrict__ cnt, const int64_t* __restrict__ coff,
+                                 const int32_t* __restrict__ crsz, const int32_t* __restrict__ ccsz, int32_t nib,
+                                 int64_t nc, int64_t* __restrict__ need, int64_t* __restrict__ outflag) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  const bool any = cnt[c] > 0;
+  const int ib = (int)(c % nib), jb = (int)(c / nib);
+  need[c] = (any && coff[c] < 0) ? (int64_t)crsz[ib] * ccsz[jb] : 0;
+  outflag[c] = any ? 1 : 0;
+}
+
+// geometry of op(X) for screening / checking
+struct OpGeo {
+  int32_t nrb, ncb;
+  const std::vector<int32_t>* rsz;
+  const std::vector<int32_t>* csz;
+  const std::vector<int32_t>* rscr;
+  const std::vector<int32_t>* cscr;
+  const int32_t* d_rscr;
+  const int32_t* d_cscr;
+  const int32_t* d_rsz;
+  const int32_t* d_csz;
+};
+
+static OpGeo op_geo(const sa_bsm* m, bool t) {
+  if (!t) return OpGeo{m->nrb, m->ncb, &m->rsz, &m->csz, &m->rscr, &m->cscr, m->d_rscr, m->d_cscr, m->d_rsz, m->d_csz};
+  return OpGeo{m->ncb, m->nrb, &m->csz, &m->rsz, &m->cscr, &m->rscr, m->d_cscr, m->d_rscr, m->d_csz, m->d_rsz};
+}
+
+void check_mm_geometry(const sa_bsm* C, const sa_bsm* A, bool ta, const sa_bsm* B, bool tb) {
+  const OpGeo ga = op_geo(A, ta), gb = op_geo(B, tb);
+  SA_REQUIRE(ga.ncb == gb.nrb && *ga.csz == *gb.rsz, "matmult: inner block dimensions of op(A) and op(B) differ");
+  SA_REQUIRE(ga.nrb == C->nrb && *ga.rsz == C->rsz && *ga.rscr == C->rscr,
+             "matmult: row blocks of C and op(A) differ");
+  SA_REQUIRE(gb.ncb == C->ncb && *gb.csz == C->csz && *gb.cscr == C->cscr,
+             "matmult: column blocks of C and op(B) differ");
+  SA_REQUIRE(C != A && C != B, "matmult: C must not alias A or B");
+  SA_REQUIRE(A->ctx == C->ctx && B->ctx == C->ctx, "matmult: matrices belong to different contexts");
+}
+
+// Screened FP64 multiply into C (after the beta pre-pass).  tree/lvl_off
+// (device, optional) exclude triples covered by executed Strassen nodes.
+// Returns the number of tasks.
+int64_t screened_multiply(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, double thresh, double alpha,
+                          int tile_cfg, const uint8_t* d_tree, const int64_t* d_lvl_off, int nlev) {
+  sa_context* c = C->ctx;
+  const OpGeo ga = op_geo(A, ta), gb = op_geo(B, tb);
+  const int32_t nib = ga.nrb, njb = gb.ncb, nkb = ga.ncb;
+  const int64_t nc = (int64_t)nib * njb;
+  c->last_nc = nc;
+  c->last_nib = nib;
+  c->last_ntasks = 0;
+  if (nc == 0) return 0;
+
+  DevBuf& bcnt = c->buf("mm_cnt");
+  DevBuf& bseg = c->buf("mm_seg");
+  DevBuf& bneed = c->buf("mm_need");
+  DevBuf& bout = c->buf("mm_out");
+  DevBuf& btot = c->buf("mm_totals");
+  bcnt.reserve(sizeof(int64_t) * (size_t)(nc + 1));
+  bseg.reserve(sizeof(int64_t) * (size_t)(nc + 1));
+  bneed.reserve(sizeof(int64_t) * (size_t)(nc + 1));
+  bout.reserve(sizeof(int64_t) * (size_t)(nc + 1));
+  btot.reserve(sizeof(int64_t) * 8);
+  int64_t* cnt = bcnt.as<int64_t>();
+  int64_t* seg = bseg.as<int64_t>();
+  int64_t* need = bneed.as<int64_t>();
+  int64_t* outf = bout.as<int64_t>();
+  int64_t* tot = btot.as<int64_t>();
+
+  ScreenArgs s;
+  s.aoff = A->d_off; s.anorm = A->d_norm; s.boff = B->d_off; s.bnorm = B->d_norm;
+  s.iscr = ga.d_rscr; s.jscr = gb.d_cscr; s.kscr = ga.d_cscr;
+  s.a_nrb = A->nrb; s.b_nrb = B->nrb;
+  s.nib = nib; s.njb = njb; s.nkb = nkb;
+  s.ta = ta; s.tb = tb;
+  s.thresh = thresh;
+  s.tree = d_tree; s.lvl_off = d_lvl_off; s.nlev = nlev;
+
+  {
+    ProfScope ps(c, "screen");
+    hipLaunchKernelGGL(k_screen<1>, dim3((unsigned)ceil_div(nc, 4)), dim3(256), 0, c->stream, s, cnt,
+                       (int32_t*)nullptr);
+  }
+  SA_CHECK_HIP(hipMemcpyAsync(seg, cnt, sizeof(int64_t) * nc, hipMemcpyDeviceToDevice, c->stream));
+  hipLaunchKernelGGL(k_need_and_flags, dim3((unsigned)ceil_div(nc, 256)), dim3(256), 0, c->stream,
+                     (const int64_t*)cnt, (const int64_t*)C->d_off, (const int32_t*)C->d_rsz,
+                     (const int32_t*)C->d_csz, nib, nc, need, outf);
+  scan_exclusive_i64_async(c, seg, nc, tot + 0);
+  scan_exclusive_i64_async(c, need, nc, tot + 1);
+  scan_exclusive_i64_async(c, outf, nc, tot + 2);
+  int64_t* h = static_cast<int64_t*>(pinned(c, sizeof(int64_t) * 3));
+  SA_CHECK_HIP(hipMemcpyAsync(h, tot, sizeof(int64_t) * 3, hipMemcpyDeviceToHost, c->stream));
+  SA_CHECK_HIP(hipStreamSynchronize(c->stream));
+  const int64_t ntasks = h[0], nnew = h[1], nouts = h[2];
+  c->last_ntasks = ntasks;
+  SA_REQUIRE(ntasks < ((int64_t)1 << 31), "matmult: task list exceeds 2^31 entries");
+
+  const int64_t cbase = C->used;
+  bsm_ensure_capacity(C, cbase + nnew);
+  C->used = cbase + nnew;
+  if (ntasks == 0) return 0;
+
+  DevBuf& bkb = c->buf("mm_kb");
+  DevBuf& bouts = c->buf("mm_outs");
+  DevBuf& bprods = c->buf("mm_prods");
+  bkb.reserve(sizeof(int32_t) * (size_t)(ntasks + 1));
+  bouts.reserve(sizeof(GemmOut) * (size_t)(nouts + 1));
+  bprods.reserve(sizeof(GemmProd) * (size_t)(ntasks + 1));
+  {
+    ProfScope ps(c, "screen");
+    hipLaunchKernelGGL(k_screen<2>, dim3((unsigned)ceil_div(nc, 4)), dim3(256), 0, c->stream, s, seg,
+                       bkb.as<int32_t>());
+  }
+  MMTables t;
+  t.cnt = cnt; t.seg = seg; t.need_scan = need; t.out_scan = outf; t.kb = bkb.as<int32_t>();
+  t.coff = C->d_off; t.cnorm = C->d_norm; t.carena = C->data(); t.crsz = C->d_rsz; t.ccsz = C->d_csz;
+  t.nib = nib; t.njb = njb; t.cbase = cbase;
+  t.aoff = A->d_off; t.aarena = A->data(); t.a_nrb = A->nrb;
+  t.boff = B->d_off; t.barena = B->data(); t.b_nrb = B->nrb;
+  t.ksz = ga.d_csz; t.ta = ta; t.tb = tb;
+  hipLaunchKernelGGL(k_build_records, dim3((unsigned)ceil_div(nc, 128)), dim3(128), 0, c->stream, t,
+                     bouts.as<GemmOut>(), bprods.as<GemmProd>());
+
+  GemmBatch gb2;
+  gb2.outs = bouts.as<GemmOut>();
+  gb2.nouts = nouts;
+  gb2.prods = bprods.as<GemmProd>();
+  gb2.nprods = ntasks;
+  gb2.ta = ta; gb2.tb = tb;
+  gb2.alpha = alpha;
+  int32_t mm = 0, mn = 0, mk = 0;
+  for (int32_t v : C->rsz) mm = std::max(mm, v);
+  for (int32_t v : C->csz) mn = std::max(mn, v);
+  for (int32_t v : *ga.csz) mk = std::max(mk, v);
+  gb2.max_m = mm; gb2.max_n = mn; gb2.max_k = mk;
+  gemm_fp64_run(c, gb2, tile_cfg);
+  return ntasks;
+}
+
+// blocksparsematrix.hpp:299-305
+void beta_prepass(sa_bsm* C, double beta) {
+  if (beta == 1.0) return;
+  const int rc = beta == 0.0 ? sa_bsm_zero(C) : sa_bsm_scale(C, beta);
+  if (rc != SA_OK) throw Error{rc};
+}
+
+}  // namespace sa
+
+using namespace sa;
+
+extern "C" {
+
+int sa_matmult(sa_bsm_t C, sa_bsm_t A, int transA, sa_bsm_t B, int transB, double thresh, double alpha, double beta,
+               const sa_opts_t* opts) {
+  return guard([&] {
+    SA_REQUIRE(C && A && B, "null matrix");
+    sa_opts_t o;
+    sa_opts_default(&o);
+    if (opts) o = *opts;
+    check_mm_geometry(C, A, transA != 0, B, transB != 0);
+    SA_CHECK_HIP(hipSetDevice(C->ctx->device));
+    beta_prepass(C, beta);
+    if (o.mode == SA_MODE_FP64) {
+      screened_multiply(C, A, transA != 0, B, transB != 0, thresh, alpha, o.tile, nullptr, nullptr, 0);
+    } else {
+      SA_UNSUPPORTED("matmult: mode %d not available", o.mode);
+    }
+  });
+}
+
+int sa_tasklist_get(sa_ctx_t c, int32_t* triples, int64_t cap, int64_t* n) {
+  return guard([&] {
+    SA_REQUIRE(c && n, "null argument");
+    *n = c->last_ntasks;
+    if (!triples || cap <= 0 || c->last_ntasks == 0) return;
+    const int64_t nc = c->last_nc, nt = c->last_ntasks;
+    std::vector<int64_t> cnt(nc), seg(nc);
+    std::vector<int32_t> kb(nt);
+    SA_CHECK_HIP(hipMemcpyAsync(cnt.data(), c->buf("mm_cnt").p, sizeof(int64_t) * nc, hipMemcpyDeviceToHost, c->stream));
+    SA_CHECK_HIP(hipMemcpyAsync(seg.data(), c->buf("mm_seg").p, sizeof(int64_t) * nc, hipMemcpyDeviceToHost, c->stream));
+    SA_CHECK_HIP(hipMemcpyAsync(kb.data(), c->buf("mm_kb").p, sizeof(int32_t) * nt, hipMemcpyDeviceToHost, c->stream));
+    SA_CHECK_HIP(hipStreamSynchronize(c->stream));
+    int64_t w = 0;
+    for (int64_t ci = 0; ci < nc && w < cap; ++ci)
+      for (int64_t p = seg[ci]; p < seg[ci] + cnt[ci] && w < cap; ++p, ++w) {
+        triples[3 * w + 0] = (int32_t)(ci % c->last_nib);
+        triples[3 * w + 1] = (int32_t)(ci / c->last_nib);
+        triples[3 * w + 2] = kb[p];
+      }
+  });
+}
+
+}  // extern "C"
