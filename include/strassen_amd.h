@@ -137,6 +137,43 @@ int sa_matmult(sa_bsm_t C, sa_bsm_t A, int transA, sa_bsm_t B, int transB, doubl
  * blocksparsematrix.hpp:353,402). */
 int sa_tasklist_get(sa_ctx_t ctx, int32_t* triples, int64_t cap, int64_t* n);
 
+/* ------------------------------------------------------- BLAS boundary -- */
+/* any_gemm<double> -> dgemm_ (blaslapack.cpp:24-27, matrix.hpp:229-278):
+ * C = alpha op(A) op(B) + beta C on host column-major arrays, computed on the
+ * device (rocBLAS). */
+int sa_dgemm(sa_ctx_t ctx, int transA, int transB, int64_t m, int64_t n, int64_t k, double alpha,
+             const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc);
+/* any_nrm2<double> -> dnrm2_ (blaslapack.cpp:34-37): the canonical norm order
+ * of the block-norm kernel. */
+int sa_dnrm2(sa_ctx_t ctx, int64_t n, const double* x, double* out);
+/* any_dot<double> -> ddot_ (blaslapack.cpp:29-32). */
+int sa_ddot(sa_ctx_t ctx, int64_t n, const double* x, const double* y, double* out);
+/* any_axpy<double> -> daxpy_ (blaslapack.cpp:12-15): y += alpha x. */
+int sa_daxpy(sa_ctx_t ctx, int64_t n, double alpha, const double* x, double* y);
+
+/* ------------------------------------------------------ hybrid Strassen -- */
+/* matmult_strassen_dryrun(A, B, thresh_per_block) -- strassen.hpp:510-679.
+ * Level bits concatenated (level 0 first, node index si + sj*ns + sk*ns^2),
+ * level_size[l] = ns^3, level_flops[0..nlevels] = per-level flop totals
+ * (index 0: sum over block triples; l+1: what strassen.hpp:676 prints).
+ * Any output pointer may be NULL except nlevels. */
+int sa_strassen_dryrun(sa_bsm_t A, sa_bsm_t B, double thresh_per_block, uint8_t* bits, int64_t cap,
+                       int64_t* level_size, uint64_t* level_flops, int32_t* nlevels);
+/* matmult_strassen_sparse(C, A, transA, B, transB, thresh, alpha, beta)
+ * -- strassen.hpp:687-753.  Square, uniform, no transposes, alpha == 1, beta
+ * ignored, C accumulated into (pre-filled by the caller; products into empty C
+ * blocks are dropped as in the reference).  opts->cutoff is
+ * min_size_for_strassen (main.cpp:9); opts->scheduler selects the decision
+ * tree (dry-run heuristic, learned selector, or caller bits in opts->tree).
+ * sa_tasklist_get then returns the screened 8-path leaf triples. */
+int sa_matmult_strassen_sparse(sa_bsm_t C, sa_bsm_t A, int transA, sa_bsm_t B, int transB, double thresh,
+                               double alpha, double beta, const sa_opts_t* opts);
+/* Statistics of the context-independent last hybrid call: executed Strassen
+ * roots, Winograd nodes, dense leaf block products, peak scratch bytes of a
+ * batch, batches, screened 8-path leaf products. */
+int sa_strassen_last_stats(int64_t* nroots, int64_t* nnodes, int64_t* nleaf_products, int64_t* scratch_bytes,
+                           int64_t* batches, int64_t* screened);
+
 #ifdef __cplusplus
 }
 #endif

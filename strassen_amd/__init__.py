@@ -8,8 +8,10 @@ headers lives in strassen_amd/cpp/.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -22,6 +24,20 @@ SLICE_FP16, SLICE_BF16 = 0, 1
 SCHED_HEURISTIC, SCHED_MLP, SCHED_TREE = 0, 1, 2
 
 _lib = None
+_live_ctx = weakref.WeakSet()
+_live_bsm = weakref.WeakSet()
+
+
+@atexit.register
+def _teardown():
+    """Release device objects before interpreter shutdown: matrices first,
+    then their contexts (a matrix refers to its context's stream)."""
+    for m in list(_live_bsm):
+        m.close()
+    for c in list(_live_ctx):
+        c.close()
+
+
 P = C.c_void_p
 I32 = C.c_int32
 I64 = C.c_int64
@@ -67,6 +83,13 @@ ABI = {
     "sa_bsm_dot": (INT, [P, P, D, P]),
     "sa_matmult": (INT, [P, P, INT, P, INT, D, D, D, P]),
     "sa_tasklist_get": (INT, [P, P, I64, P]),
+    "sa_dgemm": (INT, [P, INT, INT, I64, I64, I64, D, P, I64, P, I64, D, P, I64]),
+    "sa_dnrm2": (INT, [P, I64, P, P]),
+    "sa_ddot": (INT, [P, I64, P, P, P]),
+    "sa_daxpy": (INT, [P, I64, D, P, P]),
+    "sa_strassen_dryrun": (INT, [P, P, D, P, I64, P, P, P]),
+    "sa_matmult_strassen_sparse": (INT, [P, P, INT, P, INT, D, D, D, P]),
+    "sa_strassen_last_stats": (INT, [P, P, P, P, P, P]),
 }
 
 
@@ -122,6 +145,7 @@ class Context:
     def __init__(self, device: int = 0):
         self.h = P()
         _check(lib().sa_ctx_create(device, C.byref(self.h)))
+        _live_ctx.add(self)
 
     def close(self):
         if self.h:
@@ -183,6 +207,7 @@ class BSM:
             _check(lib().sa_bsm_create_blocked(ctx.h, len(r), _ptr(r), None if rs is None else _ptr(rs),
                                                len(c), _ptr(c), None if cs is None else _ptr(cs), thresh,
                                                C.byref(self.h)))
+        _live_bsm.add(self)
         self.nrow, self.ncol, self.nrb, self.ncb, _, _ = self.info(False)
 
     @classmethod
@@ -275,3 +300,43 @@ class BSM:
 def matmult(Cm: BSM, A: BSM, transA, B: BSM, transB, thresh, alpha=1.0, beta=0.0, opts: Opts | None = None):
     _check(lib().sa_matmult(Cm.h, A.h, int(transA), B.h, int(transB), thresh, alpha, beta,
                             None if opts is None else C.byref(opts)))
+
+
+def tree_level_sizes(nb):
+    max_level = int(np.log2(nb + 0.5))
+    return [((nb + (1 << (l + 1)) - 1) >> (l + 1)) ** 3 for l in range(max_level)]
+
+
+def strassen_dryrun(A: BSM, B: BSM, thresh_per_block):
+    """Device dry run: list of per-level bit arrays and the per-level flops."""
+    sizes = tree_level_sizes(A.nrb)
+    total = max(sum(sizes), 1)
+    bits = np.zeros(total, dtype=np.uint8)
+    lsz = np.zeros(len(sizes) + 1, dtype=np.int64)
+    lfl = np.zeros(len(sizes) + 1, dtype=np.uint64)
+    nl = I32(0)
+    _check(lib().sa_strassen_dryrun(A.h, B.h, thresh_per_block, _ptr(bits), total, _ptr(lsz), _ptr(lfl),
+                                    C.byref(nl)))
+    out, off = [], 0
+    for s in sizes:
+        out.append(bits[off:off + s].copy())
+        off += s
+    return out, lfl
+
+
+def matmult_strassen_sparse(Cm: BSM, A: BSM, B: BSM, thresh, cutoff=0, opts: Opts | None = None, tree=None):
+    o = opts if opts is not None else default_opts()
+    o.cutoff = int(cutoff)
+    if tree is not None:
+        arr = np.ascontiguousarray(np.concatenate(tree) if len(tree) else np.zeros(1), dtype=np.uint8)
+        o._tree_keep = arr
+        o.tree = arr.ctypes.data
+        o.scheduler = SCHED_TREE
+    _check(lib().sa_matmult_strassen_sparse(Cm.h, A.h, 0, B.h, 0, thresh, 1.0, 0.0, C.byref(o)))
+
+
+def strassen_last_stats():
+    v = [I64(0) for _ in range(6)]
+    _check(lib().sa_strassen_last_stats(*[C.byref(x) for x in v]))
+    keys = ["nroots", "nnodes", "nleaf_products", "scratch_bytes", "batches", "screened"]
+    return {k: x.value for k, x in zip(keys, v)}

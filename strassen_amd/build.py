@@ -62,6 +62,28 @@ def build_library(jobs: int = 8, verbose: bool = False) -> str:
     return LIB
 
 
+CPP = os.path.join(PKG, "cpp")
+BIN = os.path.join(PKG, "bin")
+DRIVERS = ["occ_ri_k_step", "3c_2c_transform", "strassen_bench"]
+
+
+def build_cpp() -> list:
+    """Drop-in C++ drivers (reference argv) linked against the HIP library."""
+    os.makedirs(BIN, exist_ok=True)
+    out = []
+
+    def one(name):
+        exe = os.path.join(BIN, name)
+        _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), "-I", CPP,
+              os.path.join(CPP, "drivers", name + ".cpp"), "-L", LIBDIR, "-lstrassen_amd",
+              "-Wl,-rpath,$ORIGIN/../lib", "-o", exe])
+        return exe
+
+    with cf.ThreadPoolExecutor(max_workers=3) as ex:
+        out = list(ex.map(one, DRIVERS))
+    return out
+
+
 def build_oracle(with_ref: bool = True) -> None:
     _run(["make", "-C", os.path.join(ROOT, "oracle"), "all"])
     if with_ref and os.path.isdir(os.environ.get("SA_REFERENCE_DIR", "/root/reference")):
@@ -74,3 +96,4 @@ def build_oracle(with_ref: bool = True) -> None:
 if __name__ == "__main__":
     build_oracle()
     print(build_library(verbose=True))
+    print(build_cpp())

@@ -11,7 +11,7 @@
 // MFMA operand layout (wave64, 16x16x4 f64):
 //   A: lane l holds A[l%16][l/16]     (row i, k index)
 //   B: lane l holds B[l/16][l%16]     (k index, column j)
-//   D: lane l, element v holds D[4*(l/16)+v][l%16]
+//   D: lane l, element v holds D[4*v + l/16][l%16]   (measured on gfx950)
 // Work items are remapped so that consecutive items (which share A/B blocks)
 // land on the same XCD (hardware dispatches workgroup i to XCD i % 8).
 #include <algorithm>
@@ -103,12 +103,12 @@ __global__ void __launch_bounds__(64 * WM * WN)
 #pragma unroll
     for (int j = 0; j < FN; ++j) {
       const int col = n0 + (wn * FN + j) * 16 + (lane & 15);
-      const int rowb = m0 + (wm * FM + i) * 16 + 4 * (lane >> 4);
+      const int rowb = m0 + (wm * FM + i) * 16 + (lane >> 4);
       if (col >= n) continue;
       double* cp = o.c + (int64_t)col * m;
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int row = rowb + v;
+        const int row = rowb + 4 * v;
         if (row < m) {
           double x = alpha * acc[i][j][v];
           if (o.acc) x = x + cp[row];

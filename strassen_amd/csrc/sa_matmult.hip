@@ -38,6 +38,9 @@ struct ScreenArgs {
   const uint8_t* tree;
   const int64_t* lvl_off;
   int32_t nlev;
+  // optional: products into an empty C block are dropped (Strassen entry,
+  // strassen.hpp:441-444 -- the leaf dgemm runs with m=n=0)
+  const int64_t* coff;
 };
 
 __device__ inline bool covered_by_strassen(const ScreenArgs& s, int ib, int jb, int kb) {
@@ -61,10 +64,11 @@ __global__ void __launch_bounds__(256) k_screen(ScreenArgs s, int64_t* __restric
   const int ib = (int)(c % s.nib), jb = (int)(c / s.nib);
   const int64_t sij = (int64_t)s.iscr[ib] * (int64_t)s.jscr[jb];
   int64_t pos = PASS == 2 ? cnt_or_seg[c] : 0;
-  for (int k0 = 0; k0 < s.nkb; k0 += 64) {
+  const int nkb = (s.coff && s.coff[c] < 0) ? 0 : s.nkb;
+  for (int k0 = 0; k0 < nkb; k0 += 64) {
     const int kb = k0 + lane;
     bool ok = false;
-    if (kb < s.nkb) {
+    if (kb < nkb) {
       const int64_t ai = s.ta ? (int64_t)kb + (int64_t)ib * s.a_nrb : (int64_t)ib + (int64_t)kb * s.a_nrb;
       const int64_t bi = s.tb ? (int64_t)jb + (int64_t)kb * s.b_nrb : (int64_t)kb + (int64_t)jb * s.b_nrb;
       if (s.aoff[ai] >= 0 && s.boff[bi] >= 0) {
@@ -190,7 +194,8 @@ void check_mm_geometry(const sa_bsm* C, const sa_bsm* A, bool ta, const sa_bsm* 
 // (device, optional) exclude triples covered by executed Strassen nodes.
 // Returns the number of tasks.
 int64_t screened_multiply(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, double thresh, double alpha,
-                          int tile_cfg, const uint8_t* d_tree, const int64_t* d_lvl_off, int nlev) {
+                          int tile_cfg, const uint8_t* d_tree, const int64_t* d_lvl_off, int nlev,
+                          bool c_must_exist) {
   sa_context* c = C->ctx;
   const OpGeo ga = op_geo(A, ta), gb = op_geo(B, tb);
   const int32_t nib = ga.nrb, njb = gb.ncb, nkb = ga.ncb;
@@ -224,6 +229,7 @@ int64_t screened_multiply(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, dou
   s.ta = ta; s.tb = tb;
   s.thresh = thresh;
   s.tree = d_tree; s.lvl_off = d_lvl_off; s.nlev = nlev;
+  s.coff = c_must_exist ? C->d_off : nullptr;
 
   {
     ProfScope ps(c, "screen");
@@ -310,7 +316,7 @@ int sa_matmult(sa_bsm_t C, sa_bsm_t A, int transA, sa_bsm_t B, int transB, doubl
     SA_CHECK_HIP(hipSetDevice(C->ctx->device));
     beta_prepass(C, beta);
     if (o.mode == SA_MODE_FP64) {
-      screened_multiply(C, A, transA != 0, B, transB != 0, thresh, alpha, o.tile, nullptr, nullptr, 0);
+      screened_multiply(C, A, transA != 0, B, transB != 0, thresh, alpha, o.tile, nullptr, nullptr, 0, false);
     } else {
       SA_UNSUPPORTED("matmult: mode %d not available", o.mode);
     }

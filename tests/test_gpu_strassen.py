@@ -1,0 +1,124 @@
+"""GPU parity of the hybrid Strassen-sparse path (strassen.hpp): dry-run
+decision trees bit-exact against the oracle, screened 8-path leaf task lists
+bit-exact, integer-valued inputs reproduce the dense product exactly (every
+FP64 operation exact), random inputs within 1e-12 relative Frobenius error of
+the oracle run on the same tree."""
+import numpy as np
+import pytest
+
+import oracle
+import strassen_amd as sa
+from strassen_amd import synth
+from gpu_util import assert_same_meta, ctx, pair_uniform, rel_fro
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def inputs(rng, nb, bs, kind, integer=False, ragged=0):
+    n = nb * bs - ragged
+    sz = synth.uniform_sizes(n, bs)
+    if integer:
+        a = rng.integers(-4, 5, (n, n)).astype(float)
+        b = rng.integers(-4, 5, (n, n)).astype(float)
+    else:
+        a = rng.standard_normal((n, n))
+        b = rng.standard_normal((n, n))
+    if kind != "dense":
+        if kind == "band":
+            s = synth.band_scales(len(sz), 3, 0.5) > 0
+        else:
+            s = rng.random((len(sz), len(sz))) < 0.5
+        rb = np.repeat(np.arange(len(sz)), sz)
+        a[~s[rb][:, rb]] = 0.0
+        b[~s.T[rb][:, rb]] = 0.0
+    return n, np.asfortranarray(a), np.asfortranarray(b)
+
+
+@pytest.mark.parametrize("nb,bs,kind", [(8, 32, "dense"), (16, 64, "dense"), (16, 64, "band"), (13, 32, "random"),
+                                        (32, 16, "random"), (9, 64, "dense")])
+def test_dryrun_bits_exact(nb, bs, kind):
+    rng = np.random.default_rng(nb * bs)
+    n, a, b = inputs(rng, nb, bs, kind, ragged=(5 if nb == 9 else 0))
+    oA, dA = pair_uniform(a, bs, bs, 1e-12)
+    oB, dB = pair_uniform(b, bs, bs, 1e-12)
+    thr = 1e-10 * bs ** 3
+    lo, fo = oracle.strassen_dryrun(oA, oB, thr)
+    ld, fd = sa.strassen_dryrun(dA, dB, thr)
+    assert len(lo) == len(ld)
+    for l, (x, y) in enumerate(zip(lo, ld)):
+        assert np.array_equal(x, y), f"level {l} differs"
+    assert np.array_equal(fo, fd)
+
+
+def run_both(a, b, n, bs, thresh, cutoff, tree=None, cfill=0.0):
+    oA, dA = pair_uniform(a, bs, bs, 0.0)
+    oB, dB = pair_uniform(b, bs, bs, 0.0)
+    oC = oracle.OBsm.uniform(n, n, bs, bs, 0.0); oC.fill(cfill)
+    dC = sa.BSM.uniform(ctx(), n, n, bs, bs, 0.0); dC.fill(cfill)
+    leaf_o, nroots_o = oracle.strassen_sparse(oC, oA, oB, thresh, cutoff, tree=tree)
+    sa.matmult_strassen_sparse(dC, dA, dB, thresh, cutoff, tree=tree)
+    st = sa.strassen_last_stats()
+    leaf_d = ctx().tasklist()
+    assert np.array_equal(leaf_o, leaf_d), f"leaf task lists differ ({len(leaf_o)} vs {len(leaf_d)})"
+    assert st["nroots"] == nroots_o
+    return oC.to_dense(), dC.to_dense(), st
+
+
+@pytest.mark.parametrize("nb,bs,cutoff,kind,levels", [
+    (8, 4, 0, "dense", {1}), (16, 4, 8, "dense", {2}), (16, 2, 0, "random", {0, 3}),
+    (32, 2, 4, "dense", {4}), (16, 4, 0, "random", {1}), (16, 8, 0, "band", {0, 2}),
+    (8, 32, 0, "dense", None), (16, 64, 512, "dense", None)])
+def test_integer_inputs_exact(nb, bs, cutoff, kind, levels):
+    rng = np.random.default_rng(nb + bs + cutoff)
+    n, a, b = inputs(rng, nb, bs, kind, integer=True)
+    tree = None if levels is None else oracle.forced_tree(nb, levels)
+    co, cd, st = run_both(a, b, n, bs, 0.0, cutoff, tree)
+    assert np.array_equal(co, a @ b)
+    assert np.array_equal(cd, a @ b)
+    if levels is None or levels:
+        assert st["nroots"] > 0
+
+
+@pytest.mark.parametrize("nb,bs,cutoff,kind,levels", [
+    (16, 64, 512, "dense", None), (16, 32, 0, "random", {1, 2}), (32, 16, 64, "band", {2}),
+    (12, 32, 0, "random", {1})])
+def test_random_inputs_vs_oracle(nb, bs, cutoff, kind, levels):
+    rng = np.random.default_rng(3 * nb + bs)
+    n, a, b = inputs(rng, nb, bs, kind)
+    tree = None if levels is None else oracle.forced_tree(nb, levels)
+    co, cd, st = run_both(a, b, n, bs, 1e-10, cutoff, tree, cfill=0.25)
+    assert rel_fro(cd, co) < TOL
+    assert rel_fro(cd, a @ b + 0.25) < 1e-12
+
+
+def test_empty_c_blocks_drop_products():
+    """C must be pre-filled; products into empty C blocks are dropped."""
+    rng = np.random.default_rng(9)
+    n, a, b = inputs(rng, 8, 32, "dense", integer=True)
+    oA, dA = pair_uniform(a, 32, 32, 0.0)
+    oB, dB = pair_uniform(b, 32, 32, 0.0)
+    c0 = np.zeros((n, n)); c0[:64, :] = 1.0
+    oC, dC = pair_uniform(c0, 32, 32, 1e-12)  # only the first two block rows exist
+    oracle.strassen_sparse(oC, oA, oB, 0.0, 0)
+    sa.matmult_strassen_sparse(dC, dA, dB, 0.0, 0)
+    assert_same_meta(oC, dC)
+    assert np.array_equal(oC.to_dense(), dC.to_dense())
+    assert np.array_equal(dC.to_dense()[:64], (a @ b + 1.0)[:64])
+
+
+def test_preconditions_rejected():
+    _, dA = pair_uniform(np.ones((64, 64)), 16, 16, 0.0)
+    _, dB = pair_uniform(np.ones((64, 64)), 16, 16, 0.0)
+    _, dC = pair_uniform(np.ones((64, 64)), 16, 16, 0.0)
+    L = sa.lib()
+    assert L.sa_matmult_strassen_sparse(dC.h, dA.h, 1, dB.h, 0, 0.0, 1.0, 0.0, None) == -1  # transpose
+    assert L.sa_matmult_strassen_sparse(dC.h, dA.h, 0, dB.h, 0, 0.0, 2.0, 0.0, None) == -1  # alpha != 1
+    _, dR = pair_uniform(np.ones((64, 48)), 16, 16, 0.0)
+    assert L.sa_matmult_strassen_sparse(dC.h, dR.h, 0, dB.h, 0, 0.0, 1.0, 0.0, None) == -1  # not square
+    # a caller tree that selects an edge super-block is rejected before any launch
+    _, dE = pair_uniform(np.ones((60, 60)), 16, 16, 0.0)
+    _, dF = pair_uniform(np.ones((60, 60)), 16, 16, 0.0)
+    _, dG = pair_uniform(np.ones((60, 60)), 16, 16, 0.0)
+    with pytest.raises(sa.SAError):
+        sa.matmult_strassen_sparse(dG, dE, dF, 0.0, 0, tree=oracle.forced_tree(4, {1}))
