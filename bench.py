@@ -115,6 +115,7 @@ def main():
     ms_total = ctx.elapsed(0, 1)
     ctx.set_profiling(False)
     gemm_ms, gemm_n = ctx.profile("gemm_fp64")
+    cfg_id, cfg_name = ctx.gemm_config()
     scr_ms, _ = ctx.profile("screen")
     ms_step = ms_total / args.steps
     dense_flops = 2.0 * n ** 3
@@ -137,7 +138,8 @@ def main():
         "data": "synthetic (seeded counter-based generator, generated in HBM)",
         "config": {"workload": w.name, "n": n, "nblocks": int(len(w.rsz)), "thresh": w.thresh,
                    "tasks": ntasks, "algorithmic_flop": flops_alg, "mode": "fp64",
-                   "gemm_tile": sa_cfg_name(args.tile), "parallelism": "single GPU"},
+                   "gemm_tile": cfg_name, "gemm_tile_choice": "forced" if args.tile >= 0 else "autotuned",
+                   "parallelism": "single GPU"},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
                      "kernel": "k_gemm_fp64", "kernel_avg_ms": round(gemm_avg_ms, 4),
@@ -153,10 +155,6 @@ def main():
                                          f"scaled by algorithmic/dense flop ratio",
                                "algorithmic_tflops": round(f_cpu / t_cpu / 1e12, 6)}
     print(json.dumps(rec))
-
-
-def sa_cfg_name(t):
-    return "heuristic" if t < 0 else str(t)
 
 
 if __name__ == "__main__":

@@ -73,6 +73,9 @@ int sa_ctx_synchronize(sa_ctx_t ctx);
 int sa_ctx_set_profiling(sa_ctx_t ctx, int on);
 int sa_ctx_profile(sa_ctx_t ctx, const char* kernel_class, double* ms, int64_t* launches);
 int sa_ctx_profile_reset(sa_ctx_t ctx);
+/* Tile configuration id and name used by the context's last FP64 GEMM launch
+ * (chosen by the autotuner unless opts->tile forced one). */
+int sa_ctx_gemm_config(sa_ctx_t ctx, int32_t* cfg, char* name, int32_t name_len);
 /* Wall time of the context stream between two sa_ctx_mark calls (ms). */
 int sa_ctx_mark(sa_ctx_t ctx, int slot);
 int sa_ctx_elapsed(sa_ctx_t ctx, int slot0, int slot1, float* ms);

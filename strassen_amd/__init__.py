@@ -61,6 +61,7 @@ ABI = {
     "sa_ctx_set_profiling": (INT, [P, INT]),
     "sa_ctx_profile": (INT, [P, C.c_char_p, P, P]),
     "sa_ctx_profile_reset": (INT, [P]),
+    "sa_ctx_gemm_config": (INT, [P, P, C.c_char_p, I32]),
     "sa_ctx_mark": (INT, [P, INT]),
     "sa_ctx_elapsed": (INT, [P, INT, INT, P]),
     "sa_bsm_create": (INT, [P, I64, I64, I32, I32, D, P]),
@@ -172,6 +173,12 @@ class Context:
 
     def profile_reset(self):
         _check(lib().sa_ctx_profile_reset(self.h))
+
+    def gemm_config(self):
+        cfg = I32(-1)
+        buf = C.create_string_buffer(64)
+        _check(lib().sa_ctx_gemm_config(self.h, C.byref(cfg), buf, 64))
+        return cfg.value, buf.value.decode()
 
     def mark(self, slot: int):
         _check(lib().sa_ctx_mark(self.h, slot))

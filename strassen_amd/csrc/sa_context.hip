@@ -286,6 +286,16 @@ int sa_ctx_profile_reset(sa_ctx_t c) {
   });
 }
 
+int sa_ctx_gemm_config(sa_ctx_t c, int32_t* cfg, char* name, int32_t name_len) {
+  return guard([&] {
+    SA_REQUIRE(c, "null context");
+    if (cfg) *cfg = c->last_gemm_cfg;
+    if (name && name_len > 0) {
+      std::snprintf(name, (size_t)name_len, "%s", c->last_gemm_cfg >= 0 ? gemm_fp64_config_name(c->last_gemm_cfg) : "");
+    }
+  });
+}
+
 int sa_ctx_mark(sa_ctx_t c, int slot) {
   return guard([&] {
     SA_REQUIRE(c && slot >= 0 && slot < 16, "bad mark slot");

@@ -15,6 +15,8 @@
 // Work items are remapped so that consecutive items (which share A/B blocks)
 // land on the same XCD (hardware dispatches workgroup i to XCD i % 8).
 #include <algorithm>
+#include <map>
+#include <tuple>
 
 #include "sa_internal.hpp"
 
@@ -118,6 +120,81 @@ __global__ void __launch_bounds__(64 * WM * WN)
     }
 }
 
+// Wave-per-tile variant for small blocks: each wave owns one output tile of
+// (16*FM) x (16*FN), loads its MFMA operand fragments straight from global
+// memory (L1/L2 resident for atom-sized blocks) and prefetches the next k-step
+// into registers while the current MFMAs issue.  No LDS and no barriers, so NW
+// independent waves share a workgroup.
+template <int FM, int FN, int NW, bool TA, bool TB>
+__global__ void __launch_bounds__(64 * NW)
+    k_gemm_fp64_wave(const GemmOut* __restrict__ outs, const GemmProd* __restrict__ prods,
+                     const GemmItem* __restrict__ items, int64_t nitems, double alpha) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t nblk = (nitems + NW - 1) / NW;
+  const int64_t it = xcd_remap(blockIdx.x, nblk) * NW + w;
+  if (it >= nitems) return;
+  const GemmItem item = items ? items[it] : GemmItem{(int32_t)it, 0, 0};
+  const GemmOut o = outs[item.out];
+  const int m = o.m, n = o.n;
+  const int m0 = item.tm * 16 * FM, n0 = item.tn * 16 * FN;
+  const int li = lane & 15, lk = lane >> 4;
+
+  dbl4 acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
+
+  for (int p = o.p0; p < o.p1; ++p) {
+    const GemmProd pr = prods[p];
+    const int k = pr.k;
+    double ac[FM], bc[FN], an[FM], bn[FN];
+    auto load = [&](int k0, double* a, double* b) {
+      const int kk = k0 + lk;
+#pragma unroll
+      for (int i = 0; i < FM; ++i) {
+        const int r = m0 + i * 16 + li;
+        a[i] = (r < m && kk < k) ? (TA ? pr.a[kk + (int64_t)r * k] : pr.a[r + (int64_t)kk * m]) : 0.0;
+      }
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        const int cc = n0 + j * 16 + li;
+        b[j] = (cc < n && kk < k) ? (TB ? pr.b[cc + (int64_t)kk * n] : pr.b[kk + (int64_t)cc * k]) : 0.0;
+      }
+    };
+    load(0, ac, bc);
+    for (int k0 = 0; k0 < k; k0 += 4) {
+      if (k0 + 4 < k) load(k0 + 4, an, bn);
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ac[i], bc[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+      for (int i = 0; i < FM; ++i) ac[i] = an[i];
+#pragma unroll
+      for (int j = 0; j < FN; ++j) bc[j] = bn[j];
+    }
+  }
+
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const int col = n0 + j * 16 + li;
+      if (col >= n) continue;
+      double* cp = o.c + (int64_t)col * m;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = m0 + i * 16 + 4 * v + lk;
+        if (row < m) {
+          double x = alpha * acc[i][j][v];
+          if (o.acc) x = x + cp[row];
+          cp[row] = x;
+        }
+      }
+    }
+}
+
 // ----------------------------------------------------------- work items --
 __global__ void k_count_tiles(const GemmOut* __restrict__ outs, int64_t nouts, int TM, int TN,
                               int64_t* __restrict__ cnt) {
@@ -143,6 +220,7 @@ struct Cfg {
   const char* name;
   int tm, tn, threads;
   GemmFn fn[4];  // [ta*2+tb]
+  int items_per_wg = 1;
 };
 
 #define SA_CFG(NAME, WM, WN, FM, FN, KT)                                                           \
@@ -150,6 +228,14 @@ struct Cfg {
     NAME, 16 * FM * WM, 16 * FN * WN, 64 * WM * WN,                                               \
         {k_gemm_fp64<WM, WN, FM, FN, KT, false, false>, k_gemm_fp64<WM, WN, FM, FN, KT, false, true>, \
          k_gemm_fp64<WM, WN, FM, FN, KT, true, false>, k_gemm_fp64<WM, WN, FM, FN, KT, true, true>} \
+  }
+
+#define SA_WCFG(NAME, FM, FN, NW)                                                                   \
+  Cfg {                                                                                             \
+    NAME, 16 * FM, 16 * FN, 64 * NW,                                                                \
+        {k_gemm_fp64_wave<FM, FN, NW, false, false>, k_gemm_fp64_wave<FM, FN, NW, false, true>,     \
+         k_gemm_fp64_wave<FM, FN, NW, true, false>, k_gemm_fp64_wave<FM, FN, NW, true, true>},      \
+        NW                                                                                          \
   }
 
 static const Cfg kCfgs[] = {
@@ -161,6 +247,12 @@ static const Cfg kCfgs[] = {
     SA_CFG("32x32_w4_k16", 2, 2, 1, 1, 16),   // 5: 32 tile, four waves
     SA_CFG("64x64_w4_k8", 2, 2, 2, 2, 8),     // 6
     SA_CFG("128x64_w4_k16", 2, 2, 4, 2, 16),  // 7
+    SA_WCFG("wave16x16_n4", 1, 1, 4),         // 8: wave per tile, direct loads
+    SA_WCFG("wave32x32_n4", 2, 2, 4),         // 9
+    SA_WCFG("wave48x48_n4", 3, 3, 4),         // 10
+    SA_WCFG("wave32x32_n2", 2, 2, 2),         // 11
+    SA_WCFG("wave64x32_n2", 4, 2, 2),         // 12
+    SA_WCFG("wave64x64_n1", 4, 4, 1),         // 13
 };
 static constexpr int kNumCfgs = (int)(sizeof(kCfgs) / sizeof(kCfgs[0]));
 
@@ -176,9 +268,7 @@ static int heuristic_cfg(const GemmBatch& b) {
   return 4;
 }
 
-void gemm_fp64_run(sa_context* c, const GemmBatch& b, int cfg) {
-  if (b.nouts == 0) return;
-  if (cfg < 0 || cfg >= kNumCfgs) cfg = heuristic_cfg(b);
+static void launch_cfg(sa_context* c, const GemmBatch& b, int cfg) {
   const Cfg& k = kCfgs[cfg];
   const GemmItem* items = nullptr;
   int64_t nitems = b.nouts;
@@ -194,10 +284,88 @@ void gemm_fp64_run(sa_context* c, const GemmBatch& b, int cfg) {
     items = c->ws_gemm_items2.as<GemmItem>();
   }
   SA_REQUIRE(nitems < (int64_t)1 << 31, "too many GEMM work items");
-  ProfScope ps(c, "gemm_fp64");
-  hipLaunchKernelGGL(k.fn[b.ta * 2 + b.tb], dim3((unsigned)nitems), dim3(k.threads), 0, c->stream, b.outs, b.prods,
+  const int64_t nwg = ceil_div(nitems, k.items_per_wg);
+  hipLaunchKernelGGL(k.fn[b.ta * 2 + b.tb], dim3((unsigned)nwg), dim3(k.threads), 0, c->stream, b.outs, b.prods,
                      items, nitems, b.alpha);
   SA_CHECK_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------- autotuner --
+// The tile configuration is chosen per shape class (transposes, largest block
+// dimensions) by timing every plausible configuration on a sample of the real
+// batch (at most 8192 outputs taken at a uniform stride) whose outputs are redirected to
+// scratch with overwrite semantics, so the caller's C is untouched.  The
+// choice is cached for the process.
+__global__ void k_redirect_outs(const GemmOut* __restrict__ in, int64_t n, int64_t stride, double* __restrict__ scratch,
+                                int64_t slot, GemmOut* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  GemmOut o = in[i * stride];
+  o.c = scratch + i * slot;
+  o.acc = 0;
+  out[i] = o;
+}
+
+static std::map<std::tuple<int, int, int, int, int>, int>& tune_cache() {
+  static std::map<std::tuple<int, int, int, int, int>, int> m;
+  return m;
+}
+
+static int round16(int x) { return (x + 15) / 16 * 16; }
+
+static int autotune(sa_context* c, const GemmBatch& b) {
+  const auto key = std::make_tuple(b.ta, b.tb, b.max_m, b.max_n, b.max_k);
+  auto it = tune_cache().find(key);
+  if (it != tune_cache().end()) return it->second;
+  const int64_t S = std::min<int64_t>(b.nouts, 8192);
+  const int64_t stride = b.nouts / S;
+  const int64_t slot = (int64_t)b.max_m * b.max_n;
+  DevBuf& to = c->buf("tune_outs");
+  DevBuf& tc = c->buf("tune_c");
+  to.reserve(sizeof(GemmOut) * (size_t)(S + 1));
+  tc.reserve(sizeof(double) * (size_t)(S * slot + 1));
+  hipLaunchKernelGGL(k_redirect_outs, dim3((unsigned)ceil_div(S, 256)), dim3(256), 0, c->stream, b.outs, S, stride,
+                     tc.as<double>(), slot, to.as<GemmOut>());
+  GemmBatch bs = b;
+  bs.outs = to.as<GemmOut>();
+  bs.nouts = S;
+  hipEvent_t e0, e1;
+  SA_CHECK_HIP(hipEventCreate(&e0));
+  SA_CHECK_HIP(hipEventCreate(&e1));
+  int best = heuristic_cfg(b);
+  float best_ms = 1e30f;
+  const int pm = round16(b.max_m), pn = round16(b.max_n);
+  for (int cfg = 0; cfg < kNumCfgs; ++cfg) {
+    const Cfg& k = kCfgs[cfg];
+    if (cfg != best && ((int64_t)k.tm * k.tn > 4LL * pm * pn || k.tm > 2 * pm + 16 || k.tn > 2 * pn + 16)) continue;
+    launch_cfg(c, bs, cfg);  // warm-up (item lists, caches)
+    float t = 1e30f;
+    for (int rep = 0; rep < 2; ++rep) {
+      SA_CHECK_HIP(hipEventRecord(e0, c->stream));
+      launch_cfg(c, bs, cfg);
+      SA_CHECK_HIP(hipEventRecord(e1, c->stream));
+      SA_CHECK_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      SA_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
+      t = std::min(t, ms);
+    }
+    if (t < best_ms) {
+      best_ms = t;
+      best = cfg;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  tune_cache()[key] = best;
+  return best;
+}
+
+void gemm_fp64_run(sa_context* c, const GemmBatch& b, int cfg) {
+  if (b.nouts == 0) return;
+  if (cfg < 0 || cfg >= kNumCfgs) cfg = autotune(c, b);
+  c->last_gemm_cfg = cfg;
+  ProfScope ps(c, "gemm_fp64");
+  launch_cfg(c, b, cfg);
 }
 
 }  // namespace sa

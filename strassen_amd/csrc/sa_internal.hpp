@@ -100,6 +100,7 @@ struct sa_context {
   // start ("mm_seg") and the kb list ("mm_kb")
   int64_t last_ntasks = 0, last_nc = 0;
   int32_t last_nib = 0;
+  int32_t last_gemm_cfg = -1;
   sa::DevBuf& buf(const char* name) { return bufs[name]; }
   // profiling
   bool profiling = false;

@@ -23,6 +23,8 @@
 namespace sa {
 
 struct ScreenArgs {
+  const int32_t* rowstart;  // CSR of op(A): present kb of row ib, ascending
+  const int32_t* rowlist;
   const int64_t* aoff;
   const double* anorm;
   const int64_t* boff;
@@ -64,14 +66,17 @@ __global__ void __launch_bounds__(256) k_screen(ScreenArgs s, int64_t* __restric
   const int ib = (int)(c % s.nib), jb = (int)(c / s.nib);
   const int64_t sij = (int64_t)s.iscr[ib] * (int64_t)s.jscr[jb];
   int64_t pos = PASS == 2 ? cnt_or_seg[c] : 0;
-  const int nkb = (s.coff && s.coff[c] < 0) ? 0 : s.nkb;
-  for (int k0 = 0; k0 < nkb; k0 += 64) {
-    const int kb = k0 + lane;
+  const int r0 = s.rowstart[ib];
+  const int r1 = (s.coff && s.coff[c] < 0) ? r0 : s.rowstart[ib + 1];
+  for (int q0 = r0; q0 < r1; q0 += 64) {
+    const int q = q0 + lane;
     bool ok = false;
-    if (kb < nkb) {
+    int kb = 0;
+    if (q < r1) {
+      kb = s.rowlist[q];
       const int64_t ai = s.ta ? (int64_t)kb + (int64_t)ib * s.a_nrb : (int64_t)ib + (int64_t)kb * s.a_nrb;
       const int64_t bi = s.tb ? (int64_t)jb + (int64_t)kb * s.b_nrb : (int64_t)kb + (int64_t)jb * s.b_nrb;
-      if (s.aoff[ai] >= 0 && s.boff[bi] >= 0) {
+      if (s.boff[bi] >= 0) {
         const double est = s.anorm[ai] * s.bnorm[bi];
         const double thr = s.thresh * (double)(sij * (int64_t)s.kscr[kb]);
         ok = est >= thr;
@@ -86,6 +91,29 @@ __global__ void __launch_bounds__(256) k_screen(ScreenArgs s, int64_t* __restric
     pos += __popcll(mask);
   }
   if (PASS == 1 && lane == 0) cnt_or_seg[c] = pos;
+}
+
+// CSR of the present blocks of op(A) by row: row ib, kb ascending.
+// A block (ib, kb) of op(A) sits at table index ib*sr + kb*sk.
+__global__ void k_row_count(const int64_t* __restrict__ aoff, int32_t nrows, int32_t nk, int64_t sr, int64_t sk,
+                            int64_t* __restrict__ cnt) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  int64_t n = 0;
+  for (int32_t k = 0; k < nk; ++k) n += aoff[r * sr + k * sk] >= 0 ? 1 : 0;
+  cnt[r] = n;
+}
+
+__global__ void k_row_fill(const int64_t* __restrict__ aoff, int32_t nrows, int32_t nk, int64_t sr, int64_t sk,
+                           const int64_t* __restrict__ start, int32_t* __restrict__ rowstart,
+                           int32_t* __restrict__ list) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r > nrows) return;
+  int64_t s = start[r];
+  rowstart[r] = (int32_t)s;
+  if (r == nrows) return;
+  for (int32_t k = 0; k < nk; ++k)
+    if (aoff[r * sr + k * sk] >= 0) list[s++] = k;
 }
 
 struct MMTables {
@@ -221,7 +249,28 @@ int64_t screened_multiply(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, dou
   int64_t* outf = bout.as<int64_t>();
   int64_t* tot = btot.as<int64_t>();
 
+  // CSR of op(A) rows (one extra slot so start[nib] is the total)
+  DevBuf& brc = c->buf("mm_rowcnt");
+  DevBuf& brs = c->buf("mm_rowstart");
+  DevBuf& brl = c->buf("mm_rowlist");
+  brc.reserve(sizeof(int64_t) * (size_t)(nib + 2));
+  brs.reserve(sizeof(int32_t) * (size_t)(nib + 2));
+  brl.reserve(sizeof(int32_t) * (size_t)((int64_t)nib * nkb + 1));
+  {
+    const int64_t sr = ta ? A->nrb : 1, sk = ta ? 1 : A->nrb;
+    int64_t* rc = brc.as<int64_t>();
+    fill_i64(c, rc + nib, 1, 0);
+    hipLaunchKernelGGL(k_row_count, dim3((unsigned)ceil_div(nib, 256)), dim3(256), 0, c->stream,
+                       (const int64_t*)A->d_off, nib, nkb, sr, sk, rc);
+    scan_exclusive_i64_async(c, rc, nib + 1, tot + 3);
+    hipLaunchKernelGGL(k_row_fill, dim3((unsigned)ceil_div(nib + 1, 256)), dim3(256), 0, c->stream,
+                       (const int64_t*)A->d_off, nib, nkb, sr, sk, (const int64_t*)rc, brs.as<int32_t>(),
+                       brl.as<int32_t>());
+  }
+
   ScreenArgs s;
+  s.rowstart = brs.as<int32_t>();
+  s.rowlist = brl.as<int32_t>();
   s.aoff = A->d_off; s.anorm = A->d_norm; s.boff = B->d_off; s.bnorm = B->d_norm;
   s.iscr = ga.d_rscr; s.jscr = gb.d_cscr; s.kscr = ga.d_cscr;
   s.a_nrb = A->nrb; s.b_nrb = B->nrb;

@@ -53,7 +53,7 @@ def test_alpha_beta_existing_c(beta):
     check_mm(oC, dC, oA, dA, 0, oB, dB, 0, 1e-4, -1.25, beta)
 
 
-@pytest.mark.parametrize("tile", list(range(8)))
+@pytest.mark.parametrize("tile", list(range(14)))
 def test_every_tile_config(tile):
     rng = np.random.default_rng(11)
     w = synth.small_uneven(n=500, fill=0.3, seed=4)
