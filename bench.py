@@ -2,12 +2,24 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 
-A step is one full screened multiply C = A*B (screening, task-list build,
-C-block allocation and the grouped MFMA GEMM) over inputs already resident in
-HBM (generated on the device from a seed).  Prints one JSON line:
-value = dense-equivalent TFLOP/s (2 n^3 per step), plus the roofline of the
-dominant kernel (grouped GEMM, timed with HIP events on its own stream) and a
-CPU baseline timed on this host.
+Default workload (the headline metric): the RI-K step (n_bf=4000, n_aux=12000,
+n_occ=200, 300 random atoms).  A step is B' = X B followed by
+K = sum_{P,i} B'^T B' -- screening, task lists, block allocation and the
+grouped matrix-core GEMMs -- on inputs resident in HBM.  It is timed in split
+precision (FP16-slice Ozaki products, FP64 screening / norms / accumulation)
+and in plain FP64; value = dense-equivalent TFLOP/s of the split-precision
+step, with both step times, their ratio and the relative error of K against
+the FP64 result in the same JSON line.  The metric's inverse square root is
+set-up, timed separately.
+
+Other workloads (--workload config1|config2|config3|config5): one screened
+block-sparse multiply C = A B per step (config3 = config1 inputs in split
+precision).
+
+Timing: HIP events on the library's stream around the timed steps; the
+dominant kernel's average launch time comes from the library's per-launch
+HIP events (same stream).  The CPU baseline is the oracle (single-threaded C
+restatement) on a bounded sample, scaled by algorithmic flops.
 """
 from __future__ import annotations
 
@@ -25,75 +37,193 @@ sys.path.insert(0, ROOT)
 import strassen_amd as sa  # noqa: E402
 from strassen_amd import synth  # noqa: E402
 
-PEAK_FP64_TFLOPS = 78.6
-PEAK_HBM_GBS = 8000.0
-
-WORKLOADS = {
-    "config2": synth.config2,
-    "config1": synth.config1,
-    "config5": synth.config5,
-}
+PEAK_FP64 = 78.6     # TFLOP/s, MI355X dense FP64 (matrix)
+PEAK_FP16 = 2500.0   # TFLOP/s, MI355X dense FP16/BF16 matrix
+METRIC = "dense-equiv TFLOP/s & RI-K step time: mixed vs FP64 GPU at <=1e-10 rel err"
 
 
-def task_flops(ctx, rsz, ksz, csz):
-    t = ctx.tasklist()
-    rsz, ksz, csz = (np.asarray(x, dtype=np.float64) for x in (rsz, ksz, csz))
-    return float((2.0 * rsz[t[:, 0]] * csz[t[:, 1]] * ksz[t[:, 2]]).sum()), len(t)
+def log(msg):
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
-def block_bytes(m):
-    return m.info()[5]
+def rel(x, ref):
+    n = np.linalg.norm(ref)
+    return float(np.linalg.norm(x - ref) / (n if n > 0 else 1.0))
 
 
-def cpu_baseline(w, budget_s=20.0):
-    """The oracle (single-threaded C restatement) on a bounded sample: the first
-    block columns of C, grown until about budget_s/4 s of work, scaled to the
-    full task count by flops."""
+def tasks_flops(t, rsz, ksz, csz):
+    r, k, c = (np.asarray(x, dtype=np.float64) for x in (rsz, ksz, csz))
+    return float((2.0 * r[t[:, 0]] * c[t[:, 1]] * k[t[:, 2]]).sum())
+
+
+def timed(ctx, fn, steps):
+    ctx.profile_reset()
+    ctx.mark(0)
+    for _ in range(steps):
+        fn()
+    ctx.mark(1)
+    ctx.synchronize()
+    return ctx.elapsed(0, 1) / steps
+
+
+# ----------------------------------------------------------------- RI-K --
+def cpu_baseline_rik(w, X, budget_s=15.0):
+    """Oracle transform on a bounded sample: P-groups 0..g-1 against all Q for
+    the first column block of B, grown until about budget_s of work."""
     import oracle
 
-    n_cols = 1
-    t_total = 0.0
-    sample_flops = 0.0
+    bcs = w.b_col_sizes
+    b0 = synth.dense_from_scales(7, w.p_sizes, bcs[:1], w.b_scale[:, :1])
+    oB = oracle.OBsm(w.p_sizes, bcs[:1], w.thresh).from_dense(b0)
+    g = 1
     while True:
-        csz = w.csz[:n_cols]
-        a = synth.dense_from_scales(1, w.rsz, w.ksz, w.a_scale)
-        b = synth.dense_from_scales(2, w.ksz, w.csz, w.b_scale)[:, : int(np.sum(csz))]
-        kw = {} if w.uniform_bs == 0 else {}
-        oA = oracle.OBsm(w.rsz, w.ksz, w.thresh).from_dense(a)
-        oB = oracle.OBsm(w.ksz, csz, w.thresh).from_dense(b)
-        oC = oracle.OBsm(w.rsz, csz, 0.0)
+        rows = int(np.sum(w.p_sizes[:g]))
+        oX = oracle.OBsm(w.p_sizes[:g], w.p_sizes, w.thresh).from_dense(np.asfortranarray(X[:rows]))
+        oC = oracle.OBsm(w.p_sizes[:g], bcs[:1], 0.0)
         t0 = time.perf_counter()
-        tasks = oracle.matmult(oC, oA, 0, oB, 0, w.thresh)
-        t_total = time.perf_counter() - t0
-        r = np.asarray(w.rsz, float); k = np.asarray(w.ksz, float); c = np.asarray(csz, float)
-        sample_flops = float((2 * r[tasks[:, 0]] * c[tasks[:, 1]] * k[tasks[:, 2]]).sum())
-        if t_total > budget_s / 4 or n_cols >= len(w.csz):
-            break
-        n_cols = min(len(w.csz), max(n_cols + 1, int(n_cols * min(8.0, budget_s / 4 / max(t_total, 1e-3)))))
-    return t_total, sample_flops, n_cols, kw
+        t = oracle.matmult(oC, oX, 0, oB, 0, w.thresh)
+        dt = time.perf_counter() - t0
+        fl = tasks_flops(t, w.p_sizes[:g], w.p_sizes, bcs[:1])
+        if dt > budget_s / 3 or g >= len(w.p_sizes):
+            return dt, fl, g, rows
+        g = min(len(w.p_sizes), max(g + 1, int(g * min(4.0, budget_s / 3 / max(dt, 1e-3)))))
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="config2", choices=sorted(WORKLOADS))
-    ap.add_argument("--tile", type=int, default=-1)
-    ap.add_argument("--no-cpu", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
-                          "n_gpus": args.gpus}))
-        sys.exit(2)
+def bench_rik(args):
+    w = synth.rik_config()
+    ctx = sa.Context(0)
+    bcs = w.b_col_sizes
+    # set-up: metric, X = M^{-1/2} (Newton-Schulz on the device), timed separately
+    log(f"RI-K geometry: {len(w.p_sizes)} P-groups, {len(w.mu_sizes)} mu-groups, i-chunks {list(w.i_chunks)}")
+    t0 = time.perf_counter()
+    M = w.metric()
+    t_metric = time.perf_counter() - t0
+    log(f"metric built on the host ({t_metric:.1f} s)")
+    dX = sa.BSM(ctx, w.p_sizes, w.p_sizes, w.thresh)
+    t0 = time.perf_counter()
+    X, ns_it, ns_res = sa.inv_sqrt(ctx, M, X_bsm=dX, tol=1e-14, want_x=not args.no_cpu)
+    t_ns = time.perf_counter() - t0
+    log(f"X = M^-1/2: {ns_it} Newton-Schulz iterations, residual {ns_res:.2e} ({t_ns:.1f} s)")
+    del M
+    dB = sa.BSM(ctx, w.p_sizes, bcs, w.thresh).generate(7, w.b_scale)
+    dBp = sa.BSM(ctx, w.p_sizes, bcs, 0.0)
+    dK = sa.BSM(ctx, w.mu_sizes, w.mu_sizes, 0.0)
+    o64 = sa.default_opts()
+    oz_t = sa.default_opts(mode=sa.MODE_OZAKI, slices=args.slices_transform)
+    oz_k = sa.default_opts(mode=sa.MODE_OZAKI, slices=args.slices_k)
 
-    w = WORKLOADS[args.workload]()
+    # algorithmic flops from the task lists (transform, then K)
+    log(f"B generated in HBM ({dB.info()[5] / 1e9:.1f} GB)")
+    sa.matmult(dBp, dX, 0, dB, 0, w.thresh, 1.0, 0.0)
+    t_tr = ctx.tasklist()
+    log(f"transform task list: {len(t_tr)} products; B' {dBp.info()[5] / 1e9:.1f} GB")
+    f_tr = tasks_flops(t_tr, w.p_sizes, w.p_sizes, bcs)
+
+    def step64():
+        sa.ri_k_step(dX, dB, dBp, dK, w.mu_sizes, w.i_chunks, w.thresh, o64, o64)
+
+    def stepmix():
+        sa.ri_k_step(dX, dB, dBp, dK, w.mu_sizes, w.i_chunks, w.thresh, oz_t, oz_k)
+
+    step64()
+    ctx.synchronize()
+    t_k = ctx.tasklist()
+    log(f"FP64 warm-up step done; K task list: {len(t_k)} products")
+    kr = np.array([p * i for i in w.i_chunks for p in w.p_sizes])
+    f_k = tasks_flops(t_k, w.mu_sizes, kr, w.mu_sizes)
+    K64 = dK.to_dense()
+    Bp64_norm = dBp.frobenius_norm()
+    for _ in range(max(args.warmup - 1, 0)):
+        step64()
+    ctx.set_profiling(True)
+    ms64 = timed(ctx, step64, args.steps)
+    log(f"FP64 step: {ms64:.1f} ms")
+    g64_ms, g64_n = ctx.profile("gemm_fp64")
+    cfg64 = ctx.gemm_config()[1]
+
+    for _ in range(args.warmup):
+        stepmix()
+    log("split-precision warm-up done")
+    msmix = timed(ctx, stepmix, args.steps)
+    log(f"split-precision step: {msmix:.1f} ms")
+    goz_ms, goz_n = ctx.profile("gemm_ozaki")
+    gsl_ms, _ = ctx.profile("ozaki_slice")
+    scr_ms, _ = ctx.profile("screen")
+    nrm_ms, _ = ctx.profile("norms")
+    ctx.set_profiling(False)
+    Kmix = dK.to_dense()
+    err_k = rel(Kmix, K64)
+
+    dense_fl = 2.0 * w.n_aux ** 2 * w.n_bf * w.n_occ + 2.0 * w.n_bf ** 2 * w.n_aux * w.n_occ
+    pairs_t = args.slices_transform * (args.slices_transform + 1) // 2
+    pairs_k = args.slices_k * (args.slices_k + 1) // 2
+    mfma_fl = pairs_t * f_tr + pairs_k * f_k       # low-precision MFMA work per step
+    oz_per_step_ms = goz_ms / args.steps
+    achieved_lp = mfma_fl / (oz_per_step_ms * 1e-3) / 1e12
+    fp64_achieved = (f_tr + f_k) / (g64_ms / args.steps * 1e-3) / 1e12
+
+    rec = {
+        "metric": METRIC,
+        "value": round(dense_fl / (msmix * 1e-3) / 1e12, 3),
+        "unit": "dense-equivalent TFLOP/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(msmix, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 with fp16-slice products",
+        "data": "synthetic (seeded geometry; B generated in HBM from a counter-based hash)",
+        "config": {"workload": w.name, "natoms": w.natoms, "n_bf": w.n_bf, "n_aux": w.n_aux, "n_occ": w.n_occ,
+                   "p_groups": int(len(w.p_sizes)), "mu_groups": int(len(w.mu_sizes)),
+                   "i_chunks": [int(x) for x in w.i_chunks], "b_block_fill": round(float((w.b_scale > 0).mean()), 4),
+                   "thresh": w.thresh, "slices_transform": args.slices_transform, "slices_k": args.slices_k,
+                   "slice_type": "fp16", "parallelism": "single GPU"},
+        "ri_k": {"step_ms_split_precision": round(msmix, 3), "step_ms_fp64": round(ms64, 3),
+                 "speedup_vs_fp64_gpu": round(ms64 / msmix, 3), "rel_err_K_vs_fp64": err_k,
+                 "dense_equiv_tflops_fp64": round(dense_fl / (ms64 * 1e-3) / 1e12, 3),
+                 "transform_tasks": int(len(t_tr)), "k_tasks": int(len(t_k)),
+                 "algorithmic_flop": f_tr + f_k, "dense_equiv_flop": dense_fl,
+                 "fp64_gemm_tflops": round(fp64_achieved, 3), "fp64_gemm_tile": cfg64,
+                 "split_ms_per_step": {"gemm_ozaki": round(oz_per_step_ms, 3),
+                                       "slicing": round(gsl_ms / args.steps, 3),
+                                       "screening": round(scr_ms / args.steps, 3),
+                                       "norms": round(nrm_ms / args.steps, 3)},
+                 "setup": {"metric_host_s": round(t_metric, 3), "inv_sqrt_s": round(t_ns, 3),
+                           "newton_schulz_iterations": ns_it, "newton_schulz_residual": ns_res}},
+        "roofline": {"bound": "mfma", "achieved": round(achieved_lp, 3), "peak": PEAK_FP16, "unit": "TFLOP/s",
+                     "frac": round(achieved_lp / PEAK_FP16, 4), "traffic": None, "kernel": "k_gemm_ozaki",
+                     "kernel_avg_ms": round(goz_ms / max(goz_n, 1), 3),
+                     "note": "achieved = executed FP16 slice-product flops (pairs x algorithmic) / kernel time; "
+                             "FP64 path k_gemm_fp64 reported in ri_k.fp64_gemm_tflops (peak 78.6)"},
+    }
+    if not args.no_cpu:
+        log("CPU baseline sample (oracle) ...")
+        dt, fl, g, rows = cpu_baseline_rik(w, X)
+        rate = fl / dt
+        est = (f_tr + f_k) / rate
+        rec["cpu_baseline"] = {"value": round(dense_fl / est / 1e12, 6), "unit": "dense-equivalent TFLOP/s",
+                               "cores": 1, "kind": "port",
+                               "sample": f"oracle screened transform of P-groups 0..{g - 1} ({rows} rows) against "
+                                         f"all of X for B's first column block: {fl:.3g} algorithmic flop in "
+                                         f"{dt:.1f} s; full step estimated from the step's algorithmic flops",
+                               "est_step_s": round(est, 1),
+                               "gpu_speedup_split_precision": round(est / (msmix * 1e-3), 1)}
+    print(json.dumps(rec))
+
+
+# -------------------------------------------------------- screened GEMM --
+def bench_gemm(args):
+    if args.workload == "config3":
+        w = synth.config1()
+        w.name = "config3_ozaki_on_config1"
+        opts = sa.default_opts(mode=sa.MODE_OZAKI, slices=args.slices_k, tile=args.tile)
+    else:
+        w = {"config1": synth.config1, "config2": synth.config2, "config5": synth.config5}[args.workload]()
+        opts = sa.default_opts(tile=args.tile)
+    ozaki = opts.mode == sa.MODE_OZAKI
     n = w.n
     ctx = sa.Context(0)
     A = sa.BSM(ctx, w.rsz, w.ksz, w.thresh).generate(1, w.a_scale)
     B = sa.BSM(ctx, w.ksz, w.csz, w.thresh).generate(2, w.b_scale)
     Cm = sa.BSM(ctx, w.rsz, w.csz, 0.0)
-    opts = sa.default_opts(tile=args.tile)
 
     def step():
         sa.matmult(Cm, A, 0, B, 0, w.thresh, 1.0, 0.0, opts)
@@ -101,60 +231,87 @@ def main():
     for _ in range(args.warmup):
         step()
     ctx.synchronize()
-    flops_alg, ntasks = task_flops(ctx, w.rsz, w.ksz, w.csz)
-
+    t = ctx.tasklist()
+    flops_alg = tasks_flops(t, w.rsz, w.ksz, w.csz)
+    err = None
+    if ozaki:
+        ref = sa.BSM(ctx, w.rsz, w.csz, 0.0)
+        sa.matmult(ref, A, 0, B, 0, w.thresh, 1.0, 0.0)
+        err = rel(Cm.to_dense(), ref.to_dense())
+        del ref
     ctx.set_profiling(True)
-    ctx.profile_reset()
-    ctx.mark(0)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    ctx.mark(1)
-    ctx.synchronize()
-    wall = time.perf_counter() - t0
-    ms_total = ctx.elapsed(0, 1)
-    ctx.set_profiling(False)
-    gemm_ms, gemm_n = ctx.profile("gemm_fp64")
-    cfg_id, cfg_name = ctx.gemm_config()
+    ms = timed(ctx, step, args.steps)
+    cls = "gemm_ozaki" if ozaki else "gemm_fp64"
+    g_ms, g_n = ctx.profile(cls)
     scr_ms, _ = ctx.profile("screen")
-    ms_step = ms_total / args.steps
-    dense_flops = 2.0 * n ** 3
-    value = dense_flops / (ms_step * 1e-3) / 1e12
-
-    gemm_avg_ms = gemm_ms / max(gemm_n, 1)
-    achieved = flops_alg / (gemm_avg_ms * 1e-3) / 1e12
+    ctx.set_profiling(False)
+    cfg_name = ctx.gemm_config()[1] if not ozaki else f"ozaki_fp16_s{args.slices_k}"
+    dense = 2.0 * n ** 3
+    g_avg = g_ms / max(g_n, 1)
+    if ozaki:
+        pairs = args.slices_k * (args.slices_k + 1) // 2
+        achieved, peak = pairs * flops_alg / (g_avg * 1e-3) / 1e12, PEAK_FP16
+    else:
+        achieved, peak = flops_alg / (g_avg * 1e-3) / 1e12, PEAK_FP64
     rec = {
-        "metric": "dense-equiv TFLOP/s & RI-K step time: mixed vs FP64 GPU at <=1e-10 rel err",
-        "value": round(value, 3),
-        "unit": "dense-equivalent TFLOP/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
+        "metric": METRIC, "value": round(dense / (ms * 1e-3) / 1e12, 3), "unit": "dense-equivalent TFLOP/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 with fp16-slice products" if ozaki else "f64",
         "data": "synthetic (seeded counter-based generator, generated in HBM)",
-        "config": {"workload": w.name, "n": n, "nblocks": int(len(w.rsz)), "thresh": w.thresh,
-                   "tasks": ntasks, "algorithmic_flop": flops_alg, "mode": "fp64",
-                   "gemm_tile": cfg_name, "gemm_tile_choice": "forced" if args.tile >= 0 else "autotuned",
-                   "parallelism": "single GPU"},
-        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
-                     "kernel": "k_gemm_fp64", "kernel_avg_ms": round(gemm_avg_ms, 4),
+        "config": {"workload": w.name, "n": n, "nblocks": int(len(w.rsz)), "thresh": w.thresh, "tasks": int(len(t)),
+                   "algorithmic_flop": flops_alg, "gemm_tile": cfg_name,
+                   "gemm_tile_choice": "forced" if args.tile >= 0 else "autotuned", "parallelism": "single GPU",
+                   "rel_err_vs_fp64": err},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
+                     "frac": round(achieved / peak, 4), "traffic": None,
+                     "kernel": "k_gemm_ozaki" if ozaki else "k_gemm_fp64", "kernel_avg_ms": round(g_avg, 4),
                      "screen_ms_per_step": round(scr_ms / args.steps, 4)},
-        "wall_s": round(wall, 3),
     }
     if not args.no_cpu:
-        t_cpu, f_cpu, ncols, _ = cpu_baseline(w)
-        rec["cpu_baseline"] = {"value": round(f_cpu / t_cpu / 1e12 * (dense_flops / flops_alg), 6),
-                               "unit": "dense-equivalent TFLOP/s", "cores": 1, "kind": "port",
-                               "sample": f"oracle screened multiply over the first {ncols} of {len(w.csz)} "
-                                         f"C block columns ({t_cpu:.1f} s, {f_cpu:.3g} algorithmic flop), "
-                                         f"scaled by algorithmic/dense flop ratio",
-                               "algorithmic_tflops": round(f_cpu / t_cpu / 1e12, 6)}
+        import oracle
+
+        ncols, dt, fl = 1, 0.0, 0.0
+        a = synth.dense_from_scales(1, w.rsz, w.ksz, w.a_scale)
+        oA = oracle.OBsm(w.rsz, w.ksz, w.thresh).from_dense(a)
+        while True:
+            csz = w.csz[:ncols]
+            b = synth.dense_from_scales(2, w.ksz, w.csz, w.b_scale)[:, : int(np.sum(csz))]
+            oB = oracle.OBsm(w.ksz, csz, w.thresh).from_dense(b)
+            oC = oracle.OBsm(w.rsz, csz, 0.0)
+            t0 = time.perf_counter()
+            tt = oracle.matmult(oC, oA, 0, oB, 0, w.thresh)
+            dt = time.perf_counter() - t0
+            fl = tasks_flops(tt, w.rsz, w.ksz, csz)
+            if dt > 5.0 or ncols >= len(w.csz):
+                break
+            ncols = min(len(w.csz), max(ncols + 1, int(ncols * min(8.0, 5.0 / max(dt, 1e-3)))))
+        est = flops_alg / (fl / dt)
+        rec["cpu_baseline"] = {"value": round(dense / est / 1e12, 6), "unit": "dense-equivalent TFLOP/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"oracle screened multiply over the first {ncols} of {len(w.csz)} C block "
+                                         f"columns ({dt:.1f} s, {fl:.3g} algorithmic flop), scaled by flops"}
     print(json.dumps(rec))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="rik", choices=["rik", "config1", "config2", "config3", "config5"])
+    ap.add_argument("--tile", type=int, default=-1)
+    ap.add_argument("--slices-transform", type=int, default=5)
+    ap.add_argument("--slices-k", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1", "n_gpus": args.gpus}))
+        sys.exit(2)
+    if args.workload == "rik":
+        bench_rik(args)
+    else:
+        bench_gemm(args)
 
 
 if __name__ == "__main__":

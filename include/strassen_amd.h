@@ -109,6 +109,13 @@ int sa_bsm_get_meta(sa_bsm_t m, uint8_t* present, double* norms);
  * strassen_amd/synth.py), then the copy-in drop rule applies, exactly as
  * sa_bsm_from_dense of the equivalent dense matrix. */
 int sa_bsm_generate(sa_bsm_t m, uint64_t seed, const double* block_scale);
+/* Extension: reshape view.  A matrix over the same block payloads, cached
+ * norms and grid index b as `base`, with new block geometry (each block keeps
+ * its element count and its column-major storage, e.g. a block of nP x (ni*nmu)
+ * read as (nP*ni) x nmu).  Views are read-only (matmult inputs, to_dense,
+ * meta, dot); they stay valid while `base` lives; destroy with sa_bsm_destroy. */
+int sa_bsm_view_reshape(sa_bsm_t base, int32_t nrb, const int32_t* rsz, const int32_t* rscr, int32_t ncb,
+                        const int32_t* csz, const int32_t* cscr, sa_bsm_t* out);
 /* fill_with_values (blocksparsematrix.hpp:84-101): all blocks, norm sqrt(r*c)*v. */
 int sa_bsm_fill(sa_bsm_t m, double v);
 /* zero() (blocksparsematrix.hpp:79-82): all blocks empty. */
@@ -131,7 +138,10 @@ int sa_bsm_dot(sa_bsm_t a, sa_bsm_t b, double thresh, double* out);
 /* ------------------------------------------------------------- multiply -- */
 /* matmult(C, A, transA, B, transB, thresh, alpha, beta)
  * -- blocksparsematrix.h:95-99 / blocksparsematrix.hpp:292-414.
- * opts may be NULL (FP64). */
+ * opts may be NULL (FP64).  opts->mode = SA_MODE_OZAKI evaluates the same
+ * screened block products in split precision (opts->slices FP16 or BF16
+ * slices per operand on the low-precision matrix cores, FP32 accumulation per
+ * k-step, FP64 recombination). */
 int sa_matmult(sa_bsm_t C, sa_bsm_t A, int transA, sa_bsm_t B, int transB, double thresh,
                double alpha, double beta, const sa_opts_t* opts);
 /* Screened task list of the context's last sa_matmult / hybrid leaf set:
@@ -153,6 +163,28 @@ int sa_dnrm2(sa_ctx_t ctx, int64_t n, const double* x, double* out);
 int sa_ddot(sa_ctx_t ctx, int64_t n, const double* x, const double* y, double* out);
 /* any_axpy<double> -> daxpy_ (blaslapack.cpp:12-15): y += alpha x. */
 int sa_daxpy(sa_ctx_t ctx, int64_t n, double alpha, const double* x, double* y);
+
+/* ------------------------------------------------------------- RI-K step -- */
+/* Extension (the reference reads M^{-1/2} as data, 3c_2c_transform.cpp:23-26):
+ * X = M^{-1/2} of a symmetric positive definite n x n host matrix M by the
+ * coupled Newton-Schulz iteration on device DGEMM, stopped when
+ * ||Z Y - I||_F/sqrt(n) < tol.  Outputs (each optional): X as a host array,
+ * X copied into the block-sparse X_bsm (its copy-in threshold applies),
+ * iteration count and the residual ||X M X - I||_F/sqrt(n). */
+int sa_inv_sqrt(sa_ctx_t ctx, int64_t n, const double* M, double* X, sa_bsm_t X_bsm, int32_t max_iter, double tol,
+                int32_t* iters, double* residual);
+/* RI-K step (tensormult form 2 then form 1, blocksparse3d_tensor.h:10-24):
+ *   B' = X B           (screened; rows P, column blocks (mu-block, i-chunk)
+ *                       with column block index ic + n_ic*mu and i fastest
+ *                       inside a block),
+ *   K  = sum_{P,i} B'(P,mu,i) B'(P,nu,i)  (screened B'^T B' over a reshape view
+ *                       of B' whose rows are (P, i) pairs).
+ * Bp receives B', K receives K (n_mu x n_mu blocks of mu_sizes).  Each
+ * product runs in the mode of its options (FP64 or split precision);
+ * screening, norms and accumulation are FP64. */
+int sa_ri_k_step(sa_bsm_t X, sa_bsm_t B, sa_bsm_t Bp, sa_bsm_t K, int32_t n_mu, const int32_t* mu_sizes,
+                 int32_t n_ic, const int32_t* i_chunks, double thresh, const sa_opts_t* opts_transform,
+                 const sa_opts_t* opts_k);
 
 /* ------------------------------------------------------ hybrid Strassen -- */
 /* matmult_strassen_dryrun(A, B, thresh_per_block) -- strassen.hpp:510-679.

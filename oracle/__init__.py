@@ -39,6 +39,7 @@ def lib():
             "or_bsm_create": (P, [I32, P, P, I32, P, P, D]),
             "or_bsm_create_uniform": (P, [I64, I64, I32, I32, D]),
             "or_bsm_clone": (P, [P]),
+            "or_bsm_view_reshape": (P, [P, I32, P, P, I32, P, P]),
             "or_bsm_destroy": (None, [P]),
             "or_bsm_nrow": (I64, [P]),
             "or_bsm_ncol": (I64, [P]),
@@ -139,6 +140,17 @@ class OBsm:
 
     def clone(self):
         return OBsm(None, None, _handle=lib().or_bsm_clone(self.h))
+
+    def view_reshape(self, rsz, csz, rscr=None, cscr=None):
+        r = np.ascontiguousarray(rsz, dtype=np.int32)
+        c = np.ascontiguousarray(csz, dtype=np.int32)
+        rs = None if rscr is None else np.ascontiguousarray(rscr, dtype=np.int32)
+        cs = None if cscr is None else np.ascontiguousarray(cscr, dtype=np.int32)
+        h = lib().or_bsm_view_reshape(self.h, len(r), _ptr(r), None if rs is None else _ptr(rs), len(c), _ptr(c),
+                                      None if cs is None else _ptr(cs))
+        if not h:
+            raise ValueError("view_reshape: geometry does not fit")
+        return OBsm(None, None, _handle=h)
 
     def from_dense(self, a):
         a = np.asfortranarray(a, dtype=np.float64)

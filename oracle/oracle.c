@@ -131,6 +131,27 @@ int64_t or_bsm_ncol(const or_bsm* m) { return m->ncol; }
 int32_t or_bsm_nrb(const or_bsm* m) { return m->nrb; }
 int32_t or_bsm_ncb(const or_bsm* m) { return m->ncb; }
 
+/* Reshape view (the device library's sa_bsm_view_reshape): same block
+ * payloads, norms and grid index, new block geometry.  Returns a copy. */
+or_bsm* or_bsm_view_reshape(const or_bsm* s, int32_t nrb, const int32_t* rsz, const int32_t* rscr, int32_t ncb,
+                            const int32_t* csz, const int32_t* cscr) {
+  if ((int64_t)nrb * ncb != (int64_t)s->nrb * s->ncb) return NULL;
+  or_bsm* m = or_bsm_create(nrb, rsz, rscr, ncb, csz, cscr, s->thresh);
+  for (int64_t b = 0; b < (int64_t)nrb * ncb; ++b) {
+    const int64_t n = (int64_t)rsz[b % nrb] * csz[b / nrb];
+    if (n != (int64_t)s->rsz[b % s->nrb] * s->csz[b / s->nrb]) {
+      or_bsm_destroy(m);
+      return NULL;
+    }
+    if (s->blk[b]) {
+      m->blk[b] = (double*)malloc(sizeof(double) * (size_t)n);
+      memcpy(m->blk[b], s->blk[b], sizeof(double) * (size_t)n);
+    }
+    m->norm[b] = s->norm[b];
+  }
+  return m;
+}
+
 /* blocksparsematrix.hpp:35-61 */
 int or_bsm_from_dense(or_bsm* m, const double* dense, int64_t ld) {
   if (ld < m->nrow) return -1;

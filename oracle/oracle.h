@@ -55,6 +55,11 @@ int64_t or_bsm_ncol(const or_bsm* m);
 int32_t or_bsm_nrb(const or_bsm* m);
 int32_t or_bsm_ncb(const or_bsm* m);
 
+/* Reshape view: copy of s with new block geometry (equal element count per
+ * block, same grid index); NULL if the geometry does not fit. */
+or_bsm* or_bsm_view_reshape(const or_bsm* s, int32_t nrb, const int32_t* rsz, const int32_t* rscr, int32_t ncb,
+                            const int32_t* csz, const int32_t* cscr);
+
 /* copy_from_input_array (blocksparsematrix.hpp:35-61): copy every block,
  * compute its norm, drop it if norm < thresh * scr_r * scr_c. */
 int  or_bsm_from_dense(or_bsm* m, const double* dense, int64_t ld);

@@ -73,6 +73,7 @@ ABI = {
     "sa_bsm_to_dense": (INT, [P, P, I64]),
     "sa_bsm_get_meta": (INT, [P, P, P]),
     "sa_bsm_generate": (INT, [P, C.c_uint64, P]),
+    "sa_bsm_view_reshape": (INT, [P, I32, P, P, I32, P, P, P]),
     "sa_bsm_fill": (INT, [P, D]),
     "sa_bsm_zero": (INT, [P]),
     "sa_bsm_scale": (INT, [P, D]),
@@ -88,6 +89,8 @@ ABI = {
     "sa_dnrm2": (INT, [P, I64, P, P]),
     "sa_ddot": (INT, [P, I64, P, P, P]),
     "sa_daxpy": (INT, [P, I64, D, P, P]),
+    "sa_inv_sqrt": (INT, [P, I64, P, P, P, I32, D, P, P]),
+    "sa_ri_k_step": (INT, [P, P, P, P, I32, P, I32, P, D, P, P]),
     "sa_strassen_dryrun": (INT, [P, P, D, P, I64, P, P, P]),
     "sa_matmult_strassen_sparse": (INT, [P, P, INT, P, INT, D, D, D, P]),
     "sa_strassen_last_stats": (INT, [P, P, P, P, P, P]),
@@ -272,6 +275,22 @@ class BSM:
         _check(lib().sa_bsm_generate(self.h, seed, _ptr(flat)))
         return self
 
+    def view_reshape(self, rsz, csz, rscr=None, cscr=None):
+        """Read-only view with new block geometry over the same block payloads."""
+        r = np.ascontiguousarray(rsz, dtype=np.int32)
+        c = np.ascontiguousarray(csz, dtype=np.int32)
+        rs = None if rscr is None else np.ascontiguousarray(rscr, dtype=np.int32)
+        cs = None if cscr is None else np.ascontiguousarray(cscr, dtype=np.int32)
+        v = BSM.__new__(BSM)
+        v.ctx = self.ctx
+        v.h = P()
+        v._base = self  # keep the base alive
+        _check(lib().sa_bsm_view_reshape(self.h, len(r), _ptr(r), None if rs is None else _ptr(rs), len(c), _ptr(c),
+                                         None if cs is None else _ptr(cs), C.byref(v.h)))
+        _live_bsm.add(v)
+        v.nrow, v.ncol, v.nrb, v.ncb, _, _ = v.info(False)
+        return v
+
     def fill(self, v):
         _check(lib().sa_bsm_fill(self.h, v))
 
@@ -347,3 +366,24 @@ def strassen_last_stats():
     _check(lib().sa_strassen_last_stats(*[C.byref(x) for x in v]))
     keys = ["nroots", "nnodes", "nleaf_products", "scratch_bytes", "batches", "screened"]
     return {k: x.value for k, x in zip(keys, v)}
+
+
+def inv_sqrt(ctx: Context, M, X_bsm: BSM | None = None, max_iter=100, tol=1e-14, want_x=True):
+    """X = M^{-1/2} on the device (coupled Newton-Schulz).  Returns (X or None,
+    iterations, residual ||XMX - I||_F/sqrt(n))."""
+    M = np.asfortranarray(M, dtype=np.float64)
+    n = M.shape[0]
+    X = np.zeros((n, n), dtype=np.float64, order="F") if want_x else None
+    it = I32(0)
+    res = D(0.0)
+    _check(lib().sa_inv_sqrt(ctx.h, n, _ptr(M), None if X is None else _ptr(X), None if X_bsm is None else X_bsm.h,
+                             max_iter, tol, C.byref(it), C.byref(res)))
+    return X, it.value, res.value
+
+
+def ri_k_step(X: BSM, B: BSM, Bp: BSM, K: BSM, mu_sizes, i_chunks, thresh, opts_t: Opts | None = None,
+              opts_k: Opts | None = None):
+    mu = np.ascontiguousarray(mu_sizes, dtype=np.int32)
+    ic = np.ascontiguousarray(i_chunks, dtype=np.int32)
+    _check(lib().sa_ri_k_step(X.h, B.h, Bp.h, K.h, len(mu), _ptr(mu), len(ic), _ptr(ic), thresh,
+                              None if opts_t is None else C.byref(opts_t), None if opts_k is None else C.byref(opts_k)))

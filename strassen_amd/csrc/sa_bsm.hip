@@ -372,7 +372,15 @@ void bsm_alloc_tables(sa_bsm* m) {
 }
 
 void bsm_ensure_capacity(sa_bsm* m, int64_t elems) {
-  m->arena.reserve_keep(sizeof(double) * (size_t)(elems > 0 ? elems : 1), m->ctx->stream);
+  const size_t want = sizeof(double) * (size_t)(elems > 0 ? elems : 1);
+  if (want <= m->arena.bytes) return;
+  if (m->used == 0) {
+    // nothing to keep: release first and allocate the exact size (no transient copy)
+    m->arena.release();
+    m->arena.reserve(want);
+    return;
+  }
+  m->arena.reserve_keep(want, m->ctx->stream);
 }
 
 void launch_block_norms(sa_bsm* m) {
@@ -414,10 +422,16 @@ static sa_bsm* make_bsm(sa_context* c, const std::vector<int32_t>& rsz, const st
 
 static void free_bsm(sa_bsm* m) {
   (void)hipFree(m->d_rsz); (void)hipFree(m->d_csz); (void)hipFree(m->d_rscr); (void)hipFree(m->d_cscr);
-  (void)hipFree(m->d_roff); (void)hipFree(m->d_coff); (void)hipFree(m->d_off); (void)hipFree(m->d_norm);
-  m->arena.release();
+  (void)hipFree(m->d_roff); (void)hipFree(m->d_coff);
+  if (!m->base) {
+    (void)hipFree(m->d_off);
+    (void)hipFree(m->d_norm);
+    m->arena.release();
+  }
   delete m;
 }
+
+#define SA_REQUIRE_WRITABLE(m) SA_REQUIRE(!(m)->base, "operation writes to a reshape view (views are read-only)")
 
 static bool same_geometry(const sa_bsm* a, const sa_bsm* b) {
   return a->rsz == b->rsz && a->csz == b->csz && a->nrow == b->nrow && a->ncol == b->ncol;
@@ -439,6 +453,27 @@ static int64_t append_blocks(sa_bsm* m, int64_t* keep, int64_t* scan_tmp, int64_
 
 // ===================================================================== ABI ==
 using namespace sa;
+
+void sa::bsm_from_dense_device(sa_bsm* m, const double* d_dense, int64_t ld) {
+  sa_context* c = m->ctx;
+  const int64_t nb = m->nb();
+  if (nb == 0) return;
+  c->ws_tmp0.reserve(sizeof(int64_t) * (size_t)(nb + 1));
+  c->ws_tmp1.reserve(sizeof(int64_t) * (size_t)(nb + 1));
+  int64_t* keep = c->ws_tmp0.as<int64_t>();
+  int64_t* scan = c->ws_tmp1.as<int64_t>();
+  // the copy replaces every block (blocksparsematrix.hpp:52-53): restart the arena
+  m->used = 0;
+  hipLaunchKernelGGL(k_dense_norms, dim3((unsigned)ceil_div(nb, 4)), dim3(256), 0, c->stream, d_dense, ld, geo(m),
+                     nb, m->thresh, m->d_norm, keep);
+  const int64_t total = append_blocks(m, keep, scan, m->d_off);
+  bsm_ensure_capacity(m, total);
+  hipLaunchKernelGGL(k_copy_dense_blocks, dim3((unsigned)nb), dim3(256), 0, c->stream, d_dense, ld, geo(m),
+                     (const int64_t*)m->d_off, m->data());
+  m->used = total;
+  SA_CHECK_HIP(hipGetLastError());
+}
+
 
 extern "C" {
 
@@ -471,6 +506,52 @@ int sa_bsm_create_blocked(sa_ctx_t c, int32_t nrb, const int32_t* rsz, const int
     for (int32_t v : r) SA_REQUIRE(v > 0, "block sizes must be positive");
     for (int32_t v : cc) SA_REQUIRE(v > 0, "block sizes must be positive");
     *out = make_bsm(c, r, rs, cc, cs, thresh);
+  });
+}
+
+int sa_bsm_view_reshape(sa_bsm_t base, int32_t nrb, const int32_t* rsz, const int32_t* rscr, int32_t ncb,
+                        const int32_t* csz, const int32_t* cscr, sa_bsm_t* out) {
+  return guard([&] {
+    SA_REQUIRE(base && out && rsz && csz, "null argument");
+    sa_bsm* root = base->base ? base->base : base;
+    SA_REQUIRE((int64_t)nrb * ncb == base->nb(), "view: block count %lld differs from %lld",
+               (long long)((int64_t)nrb * ncb), (long long)base->nb());
+    for (int32_t cb = 0; cb < ncb; ++cb)
+      for (int32_t rb = 0; rb < nrb; ++rb) {
+        const int64_t b = rb + (int64_t)cb * nrb;
+        const int64_t orb = b % base->nrb, ocb = b / base->nrb;
+        SA_REQUIRE(rsz[rb] > 0 && csz[cb] > 0, "view: block sizes must be positive");
+        SA_REQUIRE((int64_t)rsz[rb] * csz[cb] == (int64_t)base->rsz[orb] * base->csz[ocb],
+                   "view: block %lld holds %lld elements, base block holds %lld", (long long)b,
+                   (long long)((int64_t)rsz[rb] * csz[cb]), (long long)((int64_t)base->rsz[orb] * base->csz[ocb]));
+      }
+    std::vector<int32_t> r(rsz, rsz + nrb), cc(csz, csz + ncb);
+    std::vector<int32_t> rs = rscr ? std::vector<int32_t>(rscr, rscr + nrb) : r;
+    std::vector<int32_t> cs = cscr ? std::vector<int32_t>(cscr, cscr + ncb) : cc;
+    sa_context* c = base->ctx;
+    auto* m = new sa_bsm();
+    m->ctx = c;
+    m->nrb = nrb; m->ncb = ncb;
+    m->rsz = r; m->csz = cc; m->rscr = rs; m->cscr = cs;
+    m->thresh = base->thresh;
+    m->roff.resize(nrb);
+    m->coff.resize(ncb);
+    int64_t s = 0;
+    for (int32_t i = 0; i < nrb; ++i) { m->roff[i] = s; s += r[i]; }
+    m->nrow = s;
+    s = 0;
+    for (int32_t j = 0; j < ncb; ++j) { m->coff[j] = s; s += cc[j]; }
+    m->ncol = s;
+    m->base = root;
+    m->d_off = root->d_off;
+    m->d_norm = root->d_norm;
+    try {
+      bsm_upload_geometry(m);
+    } catch (...) {
+      delete m;
+      throw;
+    }
+    *out = m;
   });
 }
 
@@ -511,50 +592,34 @@ int sa_bsm_info(sa_bsm_t m, int64_t* nrow, int64_t* ncol, int32_t* nrb, int32_t*
 int sa_bsm_copy(sa_bsm_t dst, sa_bsm_t src) {
   return guard([&] {
     SA_REQUIRE(dst && src, "null matrix");
+    SA_REQUIRE_WRITABLE(dst);
     SA_REQUIRE(same_geometry(dst, src), "copy: block geometry differs");
     if (dst == src) return;
     sa_context* c = dst->ctx;
-    bsm_ensure_capacity(dst, src->used);
-    if (src->used)
-      SA_CHECK_HIP(hipMemcpyAsync(dst->data(), src->data(), sizeof(double) * (size_t)src->used,
+    const int64_t used = src->base ? src->base->used : src->used;
+    bsm_ensure_capacity(dst, used);
+    if (used)
+      SA_CHECK_HIP(hipMemcpyAsync(dst->data(), src->data(), sizeof(double) * (size_t)used,
                                   hipMemcpyDeviceToDevice, c->stream));
-    dst->used = src->used;
+    dst->used = used;
     const int64_t nb = src->nb();
     SA_CHECK_HIP(hipMemcpyAsync(dst->d_off, src->d_off, sizeof(int64_t) * nb, hipMemcpyDeviceToDevice, c->stream));
     SA_CHECK_HIP(hipMemcpyAsync(dst->d_norm, src->d_norm, sizeof(double) * nb, hipMemcpyDeviceToDevice, c->stream));
   });
 }
 
-static void from_dense_device(sa_bsm* m, const double* d_dense, int64_t ld) {
-  sa_context* c = m->ctx;
-  const int64_t nb = m->nb();
-  if (nb == 0) return;
-  c->ws_tmp0.reserve(sizeof(int64_t) * (size_t)(nb + 1));
-  c->ws_tmp1.reserve(sizeof(int64_t) * (size_t)(nb + 1));
-  int64_t* keep = c->ws_tmp0.as<int64_t>();
-  int64_t* scan = c->ws_tmp1.as<int64_t>();
-  // the copy replaces every block (blocksparsematrix.hpp:52-53): restart the arena
-  m->used = 0;
-  hipLaunchKernelGGL(k_dense_norms, dim3((unsigned)ceil_div(nb, 4)), dim3(256), 0, c->stream, d_dense, ld, geo(m),
-                     nb, m->thresh, m->d_norm, keep);
-  const int64_t total = append_blocks(m, keep, scan, m->d_off);
-  bsm_ensure_capacity(m, total);
-  hipLaunchKernelGGL(k_copy_dense_blocks, dim3((unsigned)nb), dim3(256), 0, c->stream, d_dense, ld, geo(m),
-                     (const int64_t*)m->d_off, m->data());
-  m->used = total;
-  SA_CHECK_HIP(hipGetLastError());
-}
 
 int sa_bsm_from_dense(sa_bsm_t m, const double* host, int64_t ld) {
   return guard([&] {
     SA_REQUIRE(m && (host || m->nrow * m->ncol == 0), "null argument");
+    SA_REQUIRE_WRITABLE(m);
     SA_REQUIRE(ld >= m->nrow, "ld %lld < nrow %lld", (long long)ld, (long long)m->nrow);
     if (m->nb() == 0) return;
     sa_context* c = m->ctx;
     const size_t bytes = sizeof(double) * (size_t)ld * (size_t)(m->ncol - 1) + sizeof(double) * (size_t)m->nrow;
     c->ws_stage.reserve(bytes);
     SA_CHECK_HIP(hipMemcpyAsync(c->ws_stage.p, host, bytes, hipMemcpyHostToDevice, c->stream));
-    from_dense_device(m, c->ws_stage.as<double>(), ld);
+    bsm_from_dense_device(m, c->ws_stage.as<double>(), ld);
     SA_CHECK_HIP(hipStreamSynchronize(c->stream));
   });
 }
@@ -594,6 +659,7 @@ int sa_bsm_get_meta(sa_bsm_t m, uint8_t* present, double* norms) {
 int sa_bsm_generate(sa_bsm_t m, uint64_t seed, const double* block_scale) {
   return guard([&] {
     SA_REQUIRE(m && block_scale, "null argument");
+    SA_REQUIRE_WRITABLE(m);
     const int64_t nb = m->nb();
     if (nb == 0) return;
     sa_context* c = m->ctx;
@@ -621,6 +687,7 @@ int sa_bsm_generate(sa_bsm_t m, uint64_t seed, const double* block_scale) {
 int sa_bsm_fill(sa_bsm_t m, double v) {
   return guard([&] {
     SA_REQUIRE(m, "null matrix");
+    SA_REQUIRE_WRITABLE(m);
     const int64_t nb = m->nb();
     if (nb == 0) return;
     sa_context* c = m->ctx;
@@ -643,6 +710,7 @@ int sa_bsm_fill(sa_bsm_t m, double v) {
 int sa_bsm_zero(sa_bsm_t m) {
   return guard([&] {
     SA_REQUIRE(m, "null matrix");
+    SA_REQUIRE_WRITABLE(m);
     fill_i64(m->ctx, m->d_off, m->nb(), -1);
     fill_f64(m->ctx, m->d_norm, m->nb(), 0.0);
     m->used = 0;
@@ -652,6 +720,7 @@ int sa_bsm_zero(sa_bsm_t m) {
 int sa_bsm_scale(sa_bsm_t m, double s) {
   return guard([&] {
     SA_REQUIRE(m, "null matrix");
+    SA_REQUIRE_WRITABLE(m);
     if (m->nb() == 0) return;
     hipLaunchKernelGGL(k_scale_blocks, dim3((unsigned)m->nb()), dim3(256), 0, m->ctx->stream, geo(m),
                        (const int64_t*)m->d_off, s, m->data(), m->d_norm);
@@ -661,6 +730,7 @@ int sa_bsm_scale(sa_bsm_t m, double s) {
 int sa_bsm_calc_norms(sa_bsm_t m) {
   return guard([&] {
     SA_REQUIRE(m, "null matrix");
+    SA_REQUIRE_WRITABLE(m);
     launch_block_norms(m);
   });
 }
@@ -668,6 +738,7 @@ int sa_bsm_calc_norms(sa_bsm_t m) {
 int sa_bsm_recompress(sa_bsm_t m) {
   return guard([&] {
     SA_REQUIRE(m, "null matrix");
+    SA_REQUIRE_WRITABLE(m);
     const int64_t nb = m->nb();
     if (nb == 0) return;
     hipLaunchKernelGGL(k_recompress, dim3((unsigned)ceil_div(nb, 256)), dim3(256), 0, m->ctx->stream, geo(m), nb,
@@ -706,6 +777,7 @@ static void add_impl(sa_bsm* m, sa_bsm* rhs, int mode) {
 int sa_bsm_add(sa_bsm_t m, sa_bsm_t rhs, double sign) {
   return guard([&] {
     SA_REQUIRE(m && rhs, "null matrix");
+    SA_REQUIRE_WRITABLE(m);
     SA_REQUIRE(same_geometry(m, rhs), "add: block geometry differs");
     SA_REQUIRE(m != rhs, "add: aliasing not supported");
     add_impl(m, rhs, sign < 0 ? 1 : 0);
@@ -715,6 +787,7 @@ int sa_bsm_add(sa_bsm_t m, sa_bsm_t rhs, double sign) {
 int sa_bsm_add_transpose(sa_bsm_t m, sa_bsm_t rhs) {
   return guard([&] {
     SA_REQUIRE(m && rhs, "null matrix");
+    SA_REQUIRE_WRITABLE(m);
     SA_REQUIRE(m->rsz == rhs->csz && m->csz == rhs->rsz, "add_transpose: geometry is not transposed");
     SA_REQUIRE(m != rhs, "add_transpose: aliasing not supported");
     add_impl(m, rhs, 2);

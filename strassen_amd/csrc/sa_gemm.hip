@@ -317,7 +317,10 @@ static int autotune(sa_context* c, const GemmBatch& b) {
   const auto key = std::make_tuple(b.ta, b.tb, b.max_m, b.max_n, b.max_k);
   auto it = tune_cache().find(key);
   if (it != tune_cache().end()) return it->second;
-  const int64_t S = std::min<int64_t>(b.nouts, 8192);
+  // sample size bounded by work: about 4e9 flops of products (at least 64 outputs)
+  const double per_out = 2.0 * b.max_m * b.max_n * (double)b.max_k * ((double)b.nprods / (double)b.nouts);
+  const int64_t S = std::max<int64_t>(1, std::min<int64_t>(
+                        b.nouts, std::min<int64_t>(8192, std::max<int64_t>(64, (int64_t)(4e9 / std::max(per_out, 1.0))))));
   const int64_t stride = b.nouts / S;
   const int64_t slot = (int64_t)b.max_m * b.max_n;
   DevBuf& to = c->buf("tune_outs");
@@ -338,6 +341,7 @@ static int autotune(sa_context* c, const GemmBatch& b) {
   for (int cfg = 0; cfg < kNumCfgs; ++cfg) {
     const Cfg& k = kCfgs[cfg];
     if (cfg != best && ((int64_t)k.tm * k.tn > 4LL * pm * pn || k.tm > 2 * pm + 16 || k.tn > 2 * pn + 16)) continue;
+    if (cfg != best && (int64_t)k.tm * k.tn * 16 < (int64_t)pm * pn) continue;  // tiles far smaller than blocks
     launch_cfg(c, bs, cfg);  // warm-up (item lists, caches)
     float t = 1e30f;
     for (int rep = 0; rep < 2; ++rep) {

@@ -130,8 +130,10 @@ struct sa_bsm {
   // arena
   sa::DevBuf arena;
   int64_t used = 0;  // doubles in use (bump pointer)
+  // reshape view: shares base's block table, norms and arena (read-only)
+  sa_bsm* base = nullptr;
   int64_t nb() const { return (int64_t)nrb * ncb; }
-  double* data() const { return arena.as<double>(); }
+  double* data() const { return base ? base->data() : arena.as<double>(); }
 };
 
 namespace sa {
@@ -163,6 +165,8 @@ void fill_f64(sa_context* c, double* p, int64_t n, double v);
 
 // arena management
 void bsm_ensure_capacity(sa_bsm* m, int64_t elems);
+// copy-in (blocksparsematrix.hpp:35-61) from a dense device array
+void bsm_from_dense_device(sa_bsm* m, const double* d_dense, int64_t ld);
 void bsm_upload_geometry(sa_bsm* m);
 void bsm_alloc_tables(sa_bsm* m);
 
@@ -201,6 +205,22 @@ struct GemmBatch {
   int32_t max_m = 0, max_n = 0, max_k = 0;
   double flops = 0.0;  // algorithmic flops (for autotuning statistics)
 };
+// Result of the shared screening stage of a screened multiply: per C block
+// task counts / segment starts, the canonical kb list, new-block and
+// non-empty-output scans (all device, owned by the context's "mm_*" buffers).
+struct ScreenResult {
+  int32_t nib = 0, njb = 0, nkb = 0;
+  int64_t nc = 0, ntasks = 0, nnew = 0, nouts = 0, cbase = 0;
+  const int64_t* cnt = nullptr;
+  const int64_t* seg = nullptr;
+  const int64_t* need_scan = nullptr;
+  const int64_t* out_scan = nullptr;
+  const int32_t* kb = nullptr;
+};
+ScreenResult screen_and_allocate(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, double thresh,
+                                 const uint8_t* d_tree, const int64_t* d_lvl_off, int nlev, bool c_must_exist);
+int64_t ozaki_multiply(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, double thresh, double alpha,
+                       const sa_opts_t& o);
 // Tile configurations of the FP64 MFMA kernel.
 int gemm_fp64_num_configs();
 const char* gemm_fp64_config_name(int cfg);

@@ -215,23 +215,26 @@ void check_mm_geometry(const sa_bsm* C, const sa_bsm* A, bool ta, const sa_bsm* 
   SA_REQUIRE(gb.ncb == C->ncb && *gb.csz == C->csz && *gb.cscr == C->cscr,
              "matmult: column blocks of C and op(B) differ");
   SA_REQUIRE(C != A && C != B, "matmult: C must not alias A or B");
+  SA_REQUIRE(!C->base, "matmult: C must not be a reshape view");
+  SA_REQUIRE((A->base ? A->base : A) != C && (B->base ? B->base : B) != C, "matmult: C must not alias A or B");
   SA_REQUIRE(A->ctx == C->ctx && B->ctx == C->ctx, "matmult: matrices belong to different contexts");
 }
 
-// Screened FP64 multiply into C (after the beta pre-pass).  tree/lvl_off
-// (device, optional) exclude triples covered by executed Strassen nodes.
-// Returns the number of tasks.
-int64_t screened_multiply(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, double thresh, double alpha,
-                          int tile_cfg, const uint8_t* d_tree, const int64_t* d_lvl_off, int nlev,
-                          bool c_must_exist) {
+// Screening and C-block allocation shared by the FP64 and split-precision
+// multiplies (after the beta pre-pass).  tree/lvl_off (device, optional)
+// exclude triples covered by executed Strassen nodes.
+ScreenResult screen_and_allocate(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, double thresh,
+                                 const uint8_t* d_tree, const int64_t* d_lvl_off, int nlev, bool c_must_exist) {
   sa_context* c = C->ctx;
   const OpGeo ga = op_geo(A, ta), gb = op_geo(B, tb);
   const int32_t nib = ga.nrb, njb = gb.ncb, nkb = ga.ncb;
   const int64_t nc = (int64_t)nib * njb;
+  ScreenResult R;
+  R.nib = nib; R.njb = njb; R.nkb = nkb; R.nc = nc;
   c->last_nc = nc;
   c->last_nib = nib;
   c->last_ntasks = 0;
-  if (nc == 0) return 0;
+  if (nc == 0) return R;
 
   DevBuf& bcnt = c->buf("mm_cnt");
   DevBuf& bseg = c->buf("mm_seg");
@@ -295,41 +298,60 @@ int64_t screened_multiply(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, dou
   int64_t* h = static_cast<int64_t*>(pinned(c, sizeof(int64_t) * 3));
   SA_CHECK_HIP(hipMemcpyAsync(h, tot, sizeof(int64_t) * 3, hipMemcpyDeviceToHost, c->stream));
   SA_CHECK_HIP(hipStreamSynchronize(c->stream));
-  const int64_t ntasks = h[0], nnew = h[1], nouts = h[2];
-  c->last_ntasks = ntasks;
-  SA_REQUIRE(ntasks < ((int64_t)1 << 31), "matmult: task list exceeds 2^31 entries");
+  R.ntasks = h[0];
+  R.nnew = h[1];
+  R.nouts = h[2];
+  c->last_ntasks = R.ntasks;
+  SA_REQUIRE(R.ntasks < ((int64_t)1 << 31), "matmult: task list exceeds 2^31 entries");
 
-  const int64_t cbase = C->used;
-  bsm_ensure_capacity(C, cbase + nnew);
-  C->used = cbase + nnew;
-  if (ntasks == 0) return 0;
+  R.cbase = C->used;
+  bsm_ensure_capacity(C, R.cbase + R.nnew);
+  C->used = R.cbase + R.nnew;
+  if (R.ntasks == 0) return R;
 
   DevBuf& bkb = c->buf("mm_kb");
-  DevBuf& bouts = c->buf("mm_outs");
-  DevBuf& bprods = c->buf("mm_prods");
-  bkb.reserve(sizeof(int32_t) * (size_t)(ntasks + 1));
-  bouts.reserve(sizeof(GemmOut) * (size_t)(nouts + 1));
-  bprods.reserve(sizeof(GemmProd) * (size_t)(ntasks + 1));
+  bkb.reserve(sizeof(int32_t) * (size_t)(R.ntasks + 1));
   {
     ProfScope ps(c, "screen");
     hipLaunchKernelGGL(k_screen<2>, dim3((unsigned)ceil_div(nc, 4)), dim3(256), 0, c->stream, s, seg,
                        bkb.as<int32_t>());
   }
+  R.cnt = cnt; R.seg = seg; R.need_scan = need; R.out_scan = outf; R.kb = bkb.as<int32_t>();
+  return R;
+}
+
+MMTables mm_tables(const ScreenResult& R, sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb) {
+  const OpGeo ga = op_geo(A, ta);
   MMTables t;
-  t.cnt = cnt; t.seg = seg; t.need_scan = need; t.out_scan = outf; t.kb = bkb.as<int32_t>();
+  t.cnt = R.cnt; t.seg = R.seg; t.need_scan = R.need_scan; t.out_scan = R.out_scan; t.kb = R.kb;
   t.coff = C->d_off; t.cnorm = C->d_norm; t.carena = C->data(); t.crsz = C->d_rsz; t.ccsz = C->d_csz;
-  t.nib = nib; t.njb = njb; t.cbase = cbase;
+  t.nib = R.nib; t.njb = R.njb; t.cbase = R.cbase;
   t.aoff = A->d_off; t.aarena = A->data(); t.a_nrb = A->nrb;
   t.boff = B->d_off; t.barena = B->data(); t.b_nrb = B->nrb;
   t.ksz = ga.d_csz; t.ta = ta; t.tb = tb;
-  hipLaunchKernelGGL(k_build_records, dim3((unsigned)ceil_div(nc, 128)), dim3(128), 0, c->stream, t,
+  return t;
+}
+
+int64_t screened_multiply(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, double thresh, double alpha,
+                          int tile_cfg, const uint8_t* d_tree, const int64_t* d_lvl_off, int nlev,
+                          bool c_must_exist) {
+  sa_context* c = C->ctx;
+  const ScreenResult R = screen_and_allocate(C, A, ta, B, tb, thresh, d_tree, d_lvl_off, nlev, c_must_exist);
+  if (R.ntasks == 0) return 0;
+  DevBuf& bouts = c->buf("mm_outs");
+  DevBuf& bprods = c->buf("mm_prods");
+  bouts.reserve(sizeof(GemmOut) * (size_t)(R.nouts + 1));
+  bprods.reserve(sizeof(GemmProd) * (size_t)(R.ntasks + 1));
+  const MMTables t = mm_tables(R, C, A, ta, B, tb);
+  hipLaunchKernelGGL(k_build_records, dim3((unsigned)ceil_div(R.nc, 128)), dim3(128), 0, c->stream, t,
                      bouts.as<GemmOut>(), bprods.as<GemmProd>());
 
+  const OpGeo ga = op_geo(A, ta);
   GemmBatch gb2;
   gb2.outs = bouts.as<GemmOut>();
-  gb2.nouts = nouts;
+  gb2.nouts = R.nouts;
   gb2.prods = bprods.as<GemmProd>();
-  gb2.nprods = ntasks;
+  gb2.nprods = R.ntasks;
   gb2.ta = ta; gb2.tb = tb;
   gb2.alpha = alpha;
   int32_t mm = 0, mn = 0, mk = 0;
@@ -338,7 +360,7 @@ int64_t screened_multiply(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, dou
   for (int32_t v : *ga.csz) mk = std::max(mk, v);
   gb2.max_m = mm; gb2.max_n = mn; gb2.max_k = mk;
   gemm_fp64_run(c, gb2, tile_cfg);
-  return ntasks;
+  return R.ntasks;
 }
 
 // blocksparsematrix.hpp:299-305
@@ -366,8 +388,10 @@ int sa_matmult(sa_bsm_t C, sa_bsm_t A, int transA, sa_bsm_t B, int transB, doubl
     beta_prepass(C, beta);
     if (o.mode == SA_MODE_FP64) {
       screened_multiply(C, A, transA != 0, B, transB != 0, thresh, alpha, o.tile, nullptr, nullptr, 0, false);
+    } else if (o.mode == SA_MODE_OZAKI) {
+      ozaki_multiply(C, A, transA != 0, B, transB != 0, thresh, alpha, o);
     } else {
-      SA_UNSUPPORTED("matmult: mode %d not available", o.mode);
+      SA_REQUIRE(false, "matmult: unknown mode %d", o.mode);
     }
   });
 }

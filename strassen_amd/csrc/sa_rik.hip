@@ -1,0 +1,225 @@
+// sa_rik.hip -- the RI-K step on the device.
+//
+//  * 2-centre metric inverse square root X = M^{-1/2} (the reference reads it
+//    as input data, 2c_ints_inv05_h2o.data / 3c_2c_transform.cpp:23-26): coupled
+//    Newton-Schulz iteration (N. J. Higham, Functions of Matrices, SIAM 2008,
+//    ch. 6): A = M/c with c = ||M||_F >= lambda_max, Y_0 = A, Z_0 = I,
+//        T_k = (3I - Z_k Y_k)/2,  Y_{k+1} = Y_k T_k,  Z_{k+1} = T_k Z_k,
+//    Z_k -> A^{-1/2}, X = Z/sqrt(c).  Three dense FP64 GEMMs per iteration
+//    (rocBLAS, plain library GEMMs), stopped when ||Z Y - I||_F/sqrt(n) < tol.
+//  * RI-K step: B' = X B (screened block multiply, 3c_2c_transform.cpp:56 /
+//    tensormult form 2), cached norms of B' refreshed, then
+//    K = sum_{P,i} B'(P,mu,i) B'(P,nu,i) (tensormult form 1) as one transposed
+//    screened multiply over a reshape view of B' whose rows are (P, i) pairs.
+#include <cmath>
+#include <vector>
+
+#include "sa_internal.hpp"
+
+namespace sa {
+
+void device_dgemm(sa_context* c, bool ta, bool tb, int64_t m, int64_t n, int64_t k, double alpha, const double* A,
+                  int64_t lda, const double* B, int64_t ldb, double beta, double* Cm, int64_t ldc);
+void bsm_from_dense_device(sa_bsm* m, const double* d_dense, int64_t ld);
+int64_t screened_multiply(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, double thresh, double alpha,
+                          int tile_cfg, const uint8_t* d_tree, const int64_t* d_lvl_off, int nlev,
+                          bool c_must_exist);
+void check_mm_geometry(const sa_bsm* C, const sa_bsm* A, bool ta, const sa_bsm* B, bool tb);
+void beta_prepass(sa_bsm* C, double beta);
+int64_t ozaki_multiply(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, double thresh, double alpha,
+                       const sa_opts_t& o);
+
+__global__ void k_scale_copy(const double* __restrict__ src, double* __restrict__ dst, int64_t n, double s) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i] * s;
+}
+
+__global__ void k_identity(double* __restrict__ d, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n * n) d[i] = (i % n == i / n) ? 1.0 : 0.0;
+}
+
+// T <- (3I - T)/2
+__global__ void k_ns_update(double* __restrict__ t, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * n) return;
+  const double d = (i % n == i / n) ? 3.0 : 0.0;
+  t[i] = 0.5 * (d - t[i]);
+}
+
+// per-block partial sums of squares of (T - I) or of T
+__global__ void __launch_bounds__(256) k_fro_parts(const double* __restrict__ t, int64_t n, int minus_identity,
+                                                   double* __restrict__ part) {
+  __shared__ double s[256];
+  double a = 0.0;
+  const int64_t nn = n * n;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < nn; i += (int64_t)gridDim.x * 256) {
+    const double v = t[i] - ((minus_identity && i % n == i / n) ? 1.0 : 0.0);
+    a = fma(v, v, a);
+  }
+  s[threadIdx.x] = a;
+  __syncthreads();
+  for (int o = 128; o >= 1; o >>= 1) {
+    if ((int)threadIdx.x < o) s[threadIdx.x] = s[threadIdx.x] + s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+__global__ void __launch_bounds__(256) k_sum_parts(const double* __restrict__ part, int n, double* __restrict__ out) {
+  __shared__ double s[256];
+  double a = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) a = a + part[i];
+  s[threadIdx.x] = a;
+  __syncthreads();
+  for (int o = 128; o >= 1; o >>= 1) {
+    if ((int)threadIdx.x < o) s[threadIdx.x] = s[threadIdx.x] + s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = s[0];
+}
+
+static double dev_fro(sa_context* c, const double* t, int64_t n, bool minus_identity) {
+  const int nparts = 1024;
+  DevBuf& bp = c->buf("ns_parts");
+  bp.reserve(sizeof(double) * (nparts + 2));
+  hipLaunchKernelGGL(k_fro_parts, dim3(nparts), dim3(256), 0, c->stream, t, n, minus_identity ? 1 : 0,
+                     bp.as<double>());
+  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, c->stream, (const double*)bp.as<double>(), nparts,
+                     bp.as<double>() + nparts);
+  double* h = static_cast<double*>(pinned(c, sizeof(double)));
+  SA_CHECK_HIP(hipMemcpyAsync(h, bp.as<double>() + nparts, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  SA_CHECK_HIP(hipStreamSynchronize(c->stream));
+  return std::sqrt(*h);
+}
+
+// X = M^{-1/2}; dM, dX device n x n (ld n).  Returns iterations, residual.
+static void inv_sqrt_device(sa_context* c, int64_t n, const double* dM, double* dX, int max_iter, double tol,
+                            int* iters, double* resid) {
+  const size_t nn = (size_t)n * (size_t)n;
+  DevBuf& bY = c->buf("ns_y");
+  DevBuf& bZ = c->buf("ns_z");
+  DevBuf& bT = c->buf("ns_t");
+  DevBuf& bY2 = c->buf("ns_y2");
+  DevBuf& bZ2 = c->buf("ns_z2");
+  for (DevBuf* b : {&bY, &bZ, &bT, &bY2, &bZ2}) b->reserve(sizeof(double) * nn);
+  double *Y = bY.as<double>(), *Z = bZ.as<double>(), *T = bT.as<double>(), *Y2 = bY2.as<double>(),
+         *Z2 = bZ2.as<double>();
+  const unsigned g = (unsigned)ceil_div((int64_t)nn, 256);
+  const double cn = dev_fro(c, dM, n, false);
+  SA_REQUIRE(cn > 0 && std::isfinite(cn), "inv_sqrt: metric norm is %g", cn);
+  hipLaunchKernelGGL(k_scale_copy, dim3(g), dim3(256), 0, c->stream, dM, Y, (int64_t)nn, 1.0 / cn);
+  hipLaunchKernelGGL(k_identity, dim3(g), dim3(256), 0, c->stream, Z, n);
+  int it = 0;
+  double r = 1e300;
+  {
+    ProfScope ps(c, "newton_schulz");
+    for (it = 0; it < max_iter; ++it) {
+      device_dgemm(c, false, false, n, n, n, 1.0, Z, n, Y, n, 0.0, T, n);
+      r = dev_fro(c, T, n, true) / std::sqrt((double)n);
+      if (r < tol) break;
+      hipLaunchKernelGGL(k_ns_update, dim3(g), dim3(256), 0, c->stream, T, n);
+      device_dgemm(c, false, false, n, n, n, 1.0, Y, n, T, n, 0.0, Y2, n);
+      device_dgemm(c, false, false, n, n, n, 1.0, T, n, Z, n, 0.0, Z2, n);
+      std::swap(Y, Y2);
+      std::swap(Z, Z2);
+    }
+  }
+  hipLaunchKernelGGL(k_scale_copy, dim3(g), dim3(256), 0, c->stream, (const double*)Z, dX, (int64_t)nn,
+                     1.0 / std::sqrt(cn));
+  // residual ||X M X - I||_F / sqrt(n)
+  device_dgemm(c, false, false, n, n, n, 1.0, dX, n, dM, n, 0.0, T, n);
+  device_dgemm(c, false, false, n, n, n, 1.0, T, n, dX, n, 0.0, Y2, n);
+  *resid = dev_fro(c, Y2, n, true) / std::sqrt((double)n);
+  *iters = it;
+  SA_CHECK_HIP(hipGetLastError());
+}
+
+}  // namespace sa
+
+using namespace sa;
+
+extern "C" {
+
+int sa_inv_sqrt(sa_ctx_t c, int64_t n, const double* M, double* X, sa_bsm_t X_bsm, int32_t max_iter, double tol,
+                int32_t* iters, double* residual) {
+  return guard([&] {
+    SA_REQUIRE(c && M && n > 0, "null argument");
+    SA_REQUIRE(max_iter > 0, "max_iter must be positive");
+    SA_CHECK_HIP(hipSetDevice(c->device));
+    if (X_bsm) {
+      SA_REQUIRE(X_bsm->nrow == n && X_bsm->ncol == n, "inv_sqrt: X_bsm must be n x n");
+      SA_REQUIRE(!X_bsm->base, "inv_sqrt: X_bsm must not be a view");
+    }
+    const size_t nn = (size_t)n * (size_t)n;
+    DevBuf& bM = c->buf("ns_m");
+    DevBuf& bX = c->buf("ns_x");
+    bM.reserve(sizeof(double) * nn);
+    bX.reserve(sizeof(double) * nn);
+    SA_CHECK_HIP(hipMemcpyAsync(bM.p, M, sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
+    int it = 0;
+    double r = 0.0;
+    inv_sqrt_device(c, n, bM.as<double>(), bX.as<double>(), max_iter, tol, &it, &r);
+    if (X) SA_CHECK_HIP(hipMemcpyAsync(X, bX.p, sizeof(double) * nn, hipMemcpyDeviceToHost, c->stream));
+    if (X_bsm) bsm_from_dense_device(X_bsm, bX.as<double>(), n);
+    SA_CHECK_HIP(hipStreamSynchronize(c->stream));
+    // the iteration's n x n work arrays are set-up memory: give them back
+    for (const char* nm : {"ns_m", "ns_x", "ns_y", "ns_z", "ns_t", "ns_y2", "ns_z2"}) c->buf(nm).release();
+    if (iters) *iters = it;
+    if (residual) *residual = r;
+  });
+}
+
+int sa_ri_k_step(sa_bsm_t X, sa_bsm_t B, sa_bsm_t Bp, sa_bsm_t K, int32_t n_mu, const int32_t* mu_sizes, int32_t n_ic,
+                 const int32_t* i_chunks, double thresh, const sa_opts_t* opts_transform, const sa_opts_t* opts_k) {
+  return guard([&] {
+    SA_REQUIRE(X && B && Bp && K && mu_sizes && i_chunks, "null argument");
+    SA_REQUIRE(B->ncb == n_mu * n_ic, "ri_k: B has %d column blocks, expected n_mu*n_ic = %d", B->ncb, n_mu * n_ic);
+    for (int32_t mu = 0; mu < n_mu; ++mu)
+      for (int32_t ic = 0; ic < n_ic; ++ic)
+        SA_REQUIRE(B->csz[ic + (int64_t)n_ic * mu] == mu_sizes[mu] * i_chunks[ic],
+                   "ri_k: B column block (mu %d, i-chunk %d) has width %d, expected %d", mu, ic,
+                   B->csz[ic + (int64_t)n_ic * mu], mu_sizes[mu] * i_chunks[ic]);
+    SA_REQUIRE(K->nrb == n_mu && K->ncb == n_mu, "ri_k: K must have n_mu x n_mu blocks");
+    sa_opts_t ot, ok;
+    sa_opts_default(&ot);
+    sa_opts_default(&ok);
+    if (opts_transform) ot = *opts_transform;
+    if (opts_k) ok = *opts_k;
+    SA_CHECK_HIP(hipSetDevice(B->ctx->device));
+    // transform B' = X B
+    check_mm_geometry(Bp, X, false, B, false);
+    beta_prepass(Bp, 0.0);
+    if (ot.mode == SA_MODE_OZAKI)
+      ozaki_multiply(Bp, X, false, B, false, thresh, 1.0, ot);
+    else
+      screened_multiply(Bp, X, false, B, false, thresh, 1.0, ot.tile, nullptr, nullptr, 0, false);
+    launch_block_norms(Bp);
+    // K-view of B': rows (P, i-chunk), columns mu
+    const int32_t npb = Bp->nrb;
+    std::vector<int32_t> vr((size_t)npb * n_ic), vrs((size_t)npb * n_ic);
+    for (int32_t ic = 0; ic < n_ic; ++ic)
+      for (int32_t pb = 0; pb < npb; ++pb) {
+        vr[pb + (size_t)npb * ic] = Bp->rsz[pb] * i_chunks[ic];
+        vrs[pb + (size_t)npb * ic] = Bp->rscr[pb] * i_chunks[ic];
+      }
+    std::vector<int32_t> vc(mu_sizes, mu_sizes + n_mu);
+    sa_bsm_t view = nullptr;
+    int rc = sa_bsm_view_reshape(Bp, npb * n_ic, vr.data(), vrs.data(), n_mu, vc.data(), K->cscr.data(), &view);
+    if (rc != SA_OK) throw Error{rc};
+    try {
+      check_mm_geometry(K, view, true, view, false);
+      beta_prepass(K, 0.0);
+      if (ok.mode == SA_MODE_OZAKI)
+        ozaki_multiply(K, view, true, view, false, thresh, 1.0, ok);
+      else
+        screened_multiply(K, view, true, view, false, thresh, 1.0, ok.tile, nullptr, nullptr, 0, false);
+    } catch (...) {
+      sa_bsm_destroy(view);
+      throw;
+    }
+    sa_bsm_destroy(view);
+  });
+}
+
+}  // extern "C"

@@ -742,6 +742,7 @@ int sa_matmult_strassen_sparse(sa_bsm_t C, sa_bsm_t A, int transA, sa_bsm_t B, i
     SA_REQUIRE(uniform_square(A, B), "strassen: A and B must be square with equal uniform blocking");
     SA_REQUIRE(C->rsz == A->rsz && C->csz == A->csz, "strassen: C geometry differs from A");
     SA_REQUIRE(C != A && C != B, "strassen: C must not alias A or B");
+    SA_REQUIRE(!C->base, "strassen: C must not be a reshape view");
     sa_opts_t o;
     sa_opts_default(&o);
     if (opts) o = *opts;

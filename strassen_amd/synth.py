@@ -164,3 +164,117 @@ def small_uneven(n: int = 600, fill: float = 0.3, seed: int = 7, thresh: float =
     w = config2(n=n, fill=fill, seed=seed, thresh=thresh)
     w.name = "small_uneven"
     return w
+
+
+# ------------------------------------------------------------------ RI-K --
+BF_SIZES = np.array([9, 13, 14, 18], dtype=np.int32)
+AUX_SIZES = np.array([30, 36, 40, 44, 50], dtype=np.int32)
+
+
+def _exact_sizes(rng, natoms, total, table):
+    s = rng.choice(table, size=natoms).astype(np.int64)
+    # scale to the exact total, keeping every size >= 1
+    s = np.maximum(1, np.round(s * (total / s.sum()))).astype(np.int64)
+    d = total - int(s.sum())
+    i = 0
+    while d != 0:
+        j = i % natoms
+        if d > 0:
+            s[j] += 1; d -= 1
+        elif s[j] > 1:
+            s[j] -= 1; d += 1
+        i += 1
+    return s.astype(np.int32)
+
+
+def _morton_order(pos, box):
+    q = np.clip((pos / box * 1023).astype(np.int64), 0, 1023)
+    code = np.zeros(len(pos), dtype=np.int64)
+    for b in range(10):
+        for d in range(3):
+            code |= ((q[:, d] >> b) & 1) << (3 * b + d)
+    return np.argsort(code, kind="stable")
+
+
+def _groups(sizes, target):
+    """Consecutive items grouped until each group holds >= target functions."""
+    out, cur, acc = [], [], 0
+    for i, s in enumerate(sizes):
+        cur.append(i); acc += int(s)
+        if acc >= target:
+            out.append(cur); cur, acc = [], 0
+    if cur:
+        if out and acc < target // 2:
+            out[-1].extend(cur)
+        else:
+            out.append(cur)
+    return out
+
+
+@dataclass
+class RIKWorkload:
+    name: str
+    natoms: int
+    n_bf: int
+    n_aux: int
+    n_occ: int
+    p_sizes: np.ndarray       # aux functions per P-group (row blocks of B, X)
+    mu_sizes: np.ndarray      # basis functions per mu-group
+    i_chunks: np.ndarray      # occupied-index chunks
+    b_scale: np.ndarray       # (n_pgroups, n_mugroups * n_ic) block scales of B, column block ic + n_ic*mu
+    aux_pos: np.ndarray       # (n_aux, 3) positions of the aux functions (P-group order)
+    alpha: float
+    ridge: float
+    thresh: float
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def b_col_sizes(self):
+        return np.array([int(m) * int(i) for m in self.mu_sizes for i in self.i_chunks], dtype=np.int32)
+
+    def metric(self):
+        """M_PQ = exp(-alpha |r_P - r_Q|^2) + ridge delta_PQ (SPD), column-major."""
+        r = self.aux_pos
+        sq = (r * r).sum(1)
+        d2 = np.maximum(sq[:, None] + sq[None, :] - 2.0 * (r @ r.T), 0.0)
+        M = np.exp(-self.alpha * d2)
+        M[np.diag_indices_from(M)] += self.ridge
+        return np.asfortranarray(M)
+
+
+def rik_config(natoms=300, n_bf=4000, n_aux=12000, n_occ=200, occ_chunk=50, p_group=160, mu_group=100,
+               fill=0.35, decay=0.3, alpha=0.5, ridge=0.05, jitter=0.35, thresh=1e-10, seed=4,
+               name="config4_ri_k_step") -> RIKWorkload:
+    """RI-K step: 300 random atoms, n_bf = 4000, n_aux = 12000, n_occ = 200.
+    Atoms are ordered along a Morton curve and grouped into P-groups of about
+    p_group aux functions and mu-groups of about mu_group basis functions; B
+    block (P-group, mu-group) is present when the closest atom pair of the two
+    groups lies within the radius that gives the requested block fill, with
+    scale exp(-decay d_min)."""
+    rng = np.random.default_rng(seed)
+    box = (natoms / 0.05) ** (1.0 / 3.0)
+    pos = rng.uniform(0.0, box, size=(natoms, 3))
+    pos = pos[_morton_order(pos, box)]
+    bf = _exact_sizes(rng, natoms, n_bf, BF_SIZES)
+    aux = _exact_sizes(rng, natoms, n_aux, AUX_SIZES)
+    pg = _groups(aux, p_group)
+    mg = _groups(bf, mu_group)
+    p_sizes = np.array([int(aux[g].sum()) for g in pg], dtype=np.int32)
+    mu_sizes = np.array([int(bf[g].sum()) for g in mg], dtype=np.int32)
+    d_atoms = np.sqrt(((pos[:, None, :] - pos[None, :, :]) ** 2).sum(-1))
+    dmin = np.array([[d_atoms[np.ix_(a, b)].min() for b in mg] for a in pg])
+    rc = float(np.quantile(dmin.ravel(), fill))
+    s = np.exp(-decay * dmin)
+    s[dmin > rc] = 0.0
+    n_ic = -(-n_occ // occ_chunk)
+    i_chunks = np.array([min(occ_chunk, n_occ - i * occ_chunk) for i in range(n_ic)], dtype=np.int32)
+    b_scale = np.repeat(s, n_ic, axis=1)  # column block ic + n_ic*mu
+    aux_pos = np.repeat(pos, aux, axis=0) + rng.normal(0.0, jitter, size=(n_aux, 3))
+    return RIKWorkload(name, natoms, n_bf, n_aux, n_occ, p_sizes, mu_sizes, i_chunks, b_scale, aux_pos, alpha,
+                       ridge, thresh, extra={"rc": rc, "box": box, "fill": fill, "decay": decay})
+
+
+def rik_small(seed=5):
+    """A scaled-down RI-K geometry for parity tests (all sizes small)."""
+    return rik_config(natoms=24, n_bf=240, n_aux=600, n_occ=24, occ_chunk=8, p_group=60, mu_group=40, fill=0.5,
+                      seed=seed, name="rik_small")

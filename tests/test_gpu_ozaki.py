@@ -1,0 +1,100 @@
+"""GPU split-precision (Ozaki) screened multiply: the screened task list is
+the FP64 one (bit-exact vs the oracle) and C is within 1e-10 relative
+Frobenius error of the FP64 oracle, overall and per block row (bar stated per
+assert).  Integer-valued inputs whose bits fit the leading slice are exact."""
+import numpy as np
+import pytest
+
+import oracle
+import strassen_amd as sa
+from strassen_amd import synth
+from gpu_util import assert_same_meta, ctx, empty_pair_blocked, empty_pair_uniform, pair_blocked, pair_uniform, rel_fro
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10  # split-precision bar vs the FP64 oracle
+
+
+def block_row_errors(got, ref, rsz):
+    edges = np.concatenate([[0], np.cumsum(rsz)])
+    out = []
+    for i in range(len(rsz)):
+        r = ref[edges[i]:edges[i + 1]]
+        n = np.linalg.norm(r)
+        if n > 0:
+            out.append(np.linalg.norm(got[edges[i]:edges[i + 1]] - r) / n)
+    return np.array(out)
+
+
+def run(oA, dA, ta, oB, dB, tb, oC, dC, thresh, opts, alpha=1.0, beta=0.0):
+    t_o = oracle.matmult(oC, oA, ta, oB, tb, thresh, alpha, beta)
+    sa.matmult(dC, dA, ta, dB, tb, thresh, alpha, beta, opts)
+    assert np.array_equal(t_o, ctx().tasklist())
+    assert_same_meta(oC, dC)
+    return oC.to_dense(), dC.to_dense()
+
+
+@pytest.mark.parametrize("slices,stype", [(4, sa.SLICE_FP16), (5, sa.SLICE_FP16), (5, sa.SLICE_BF16),
+                                          (6, sa.SLICE_BF16)])
+def test_config1_like_error_per_block_row(slices, stype):
+    cfg = synth.config1(n=1024, bs=64)
+    a = synth.dense_from_scales(1, cfg.rsz, cfg.ksz, cfg.a_scale)
+    b = synth.dense_from_scales(2, cfg.ksz, cfg.csz, cfg.b_scale)
+    oA, dA = pair_uniform(a, 64, 64, cfg.thresh)
+    oB, dB = pair_uniform(b, 64, 64, cfg.thresh)
+    oC, dC = empty_pair_uniform(1024, 1024, 64, 64)
+    ref, got = run(oA, dA, 0, oB, dB, 0, oC, dC, cfg.thresh, sa.default_opts(mode=sa.MODE_OZAKI, slices=slices,
+                                                                               slice_type=stype))
+    err = rel_fro(got, ref)
+    rows = block_row_errors(got, ref, cfg.rsz)
+    bound = TOL if (stype == sa.SLICE_FP16 or slices >= 5) else 1e-8
+    assert err < bound, err
+    assert rows.max() < bound, rows.max()
+
+
+@pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
+def test_uneven_transposes(ta, tb):
+    w = synth.small_uneven(n=400, fill=0.4, seed=8)
+    a = synth.dense_from_scales(1, w.rsz, w.ksz, w.a_scale)
+    b = synth.dense_from_scales(2, w.ksz, w.csz, w.b_scale)
+    if ta:
+        a = np.asfortranarray(a.T)
+    if tb:
+        b = np.asfortranarray(b.T)
+    oA, dA = pair_blocked(a, w.ksz if ta else w.rsz, w.rsz if ta else w.ksz, w.thresh)
+    oB, dB = pair_blocked(b, w.csz if tb else w.ksz, w.ksz if tb else w.csz, w.thresh)
+    oC, dC = empty_pair_blocked(w.rsz, w.csz)
+    ref, got = run(oA, dA, ta, oB, dB, tb, oC, dC, w.thresh, sa.default_opts(mode=sa.MODE_OZAKI, slices=4))
+    assert rel_fro(got, ref) < TOL
+
+
+def test_integer_inputs_exact():
+    """Inputs with at most 3 significant bits per row max live in the leading
+    slice, so every dropped slice product is zero and the result is exact."""
+    rng = np.random.default_rng(3)
+    a = rng.integers(-4, 5, (96, 80)).astype(float)
+    b = rng.integers(-4, 5, (80, 112)).astype(float)
+    oA, dA = pair_uniform(a, 16, 16, 0.0)
+    oB, dB = pair_uniform(b, 16, 16, 0.0)
+    oC, dC = empty_pair_uniform(96, 112, 16, 16)
+    ref, got = run(oA, dA, 0, oB, dB, 0, oC, dC, 0.0, sa.default_opts(mode=sa.MODE_OZAKI, slices=4),
+                   alpha=1.0, beta=0.0)
+    assert np.array_equal(got, a @ b)
+
+
+def test_existing_c_and_alpha():
+    rng = np.random.default_rng(4)
+    a = rng.standard_normal((100, 70)); b = rng.standard_normal((70, 90)); c0 = rng.standard_normal((100, 90))
+    oA, dA = pair_uniform(a, 20, 20, 0.0)
+    oB, dB = pair_uniform(b, 20, 20, 0.0)
+    oC, dC = pair_uniform(c0, 20, 20, 0.0)
+    ref, got = run(oA, dA, 0, oB, dB, 0, oC, dC, 0.0, sa.default_opts(mode=sa.MODE_OZAKI, slices=4),
+                   alpha=-0.75, beta=1.0)
+    assert rel_fro(got, ref) < TOL
+
+
+def test_unsupported_slice_count_rejected():
+    _, dA = pair_uniform(np.ones((32, 32)), 16, 16, 0.0)
+    _, dB = pair_uniform(np.ones((32, 32)), 16, 16, 0.0)
+    _, dC = empty_pair_uniform(32, 32, 16, 16)
+    with pytest.raises(sa.SAError):
+        sa.matmult(dC, dA, 0, dB, 0, 0.0, 1.0, 0.0, sa.default_opts(mode=sa.MODE_OZAKI, slices=9))
