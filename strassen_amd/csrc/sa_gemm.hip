@@ -120,6 +120,141 @@ __global__ void __launch_bounds__(64 * WM * WN)
     }
 }
 
+// Pipelined variant for large blocks: the workgroup walks the flattened stream
+// of (product, k-slice) stages of its output tile; while the MFMAs of stage s
+// read LDS buffer s%2, the global loads of stage s+1 are already in flight into
+// registers and land in the other buffer afterwards -- one barrier per stage.
+// Rows of the staging buffers carry one pad element, so the k-major staging
+// writes of transposed operands hit distinct LDS banks.
+template <int WM, int WN, int FM, int FN, int KT, bool TA, bool TB>
+__global__ void __launch_bounds__(64 * WM * WN)
+    k_gemm_fp64_pipe(const GemmOut* __restrict__ outs, const GemmProd* __restrict__ prods,
+                     const GemmItem* __restrict__ items, int64_t nitems, double alpha) {
+  constexpr int NT = 64 * WM * WN;
+  constexpr int TM = 16 * FM * WM;
+  constexpr int TN = 16 * FN * WN;
+  constexpr int LDA = TM + 1;
+  constexpr int LDB = TN + 1;
+  constexpr int LA = (TM * KT) / NT;
+  constexpr int LB = (TN * KT) / NT;
+  static_assert((TM * KT) % NT == 0 && (TN * KT) % NT == 0, "tile/threads mismatch");
+  __shared__ double As[2][KT][LDA];
+  __shared__ double Bs[2][KT][LDB];
+
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int wm = wid % WM, wn = wid / WM;
+  const int64_t it = xcd_remap(blockIdx.x, nitems);
+  const GemmItem item = items ? items[it] : GemmItem{(int32_t)it, 0, 0};
+  const GemmOut o = outs[item.out];
+  const int m = o.m, n = o.n;
+  const int m0 = item.tm * TM, n0 = item.tn * TN;
+
+  dbl4 acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
+
+  double ra[LA], rb[LB];
+  auto gload = [&](const GemmProd& pr, int k0) {
+    const int k = pr.k;
+#pragma unroll
+    for (int r = 0; r < LA; ++r) {
+      const int e = tid + r * NT;
+      const int i = TA ? e / KT : e % TM;
+      const int kk = TA ? e % KT : e / TM;
+      const int gi = m0 + i, gk = k0 + kk;
+      ra[r] = (gi < m && gk < k) ? (TA ? pr.a[gk + (int64_t)gi * k] : pr.a[gi + (int64_t)gk * m]) : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < LB; ++r) {
+      const int e = tid + r * NT;
+      const int j = TB ? e % TN : e / KT;
+      const int kk = TB ? e / TN : e % KT;
+      const int gj = n0 + j, gk = k0 + kk;
+      rb[r] = (gj < n && gk < k) ? (TB ? pr.b[gj + (int64_t)gk * n] : pr.b[gk + (int64_t)gj * k]) : 0.0;
+    }
+  };
+  auto lstore = [&](int buf) {
+#pragma unroll
+    for (int r = 0; r < LA; ++r) {
+      const int e = tid + r * NT;
+      const int i = TA ? e / KT : e % TM;
+      const int kk = TA ? e % KT : e / TM;
+      As[buf][kk][i] = ra[r];
+    }
+#pragma unroll
+    for (int r = 0; r < LB; ++r) {
+      const int e = tid + r * NT;
+      const int j = TB ? e % TN : e / KT;
+      const int kk = TB ? e / TN : e % KT;
+      Bs[buf][kk][j] = rb[r];
+    }
+  };
+
+  int p = o.p0, k0 = 0;
+  if (p >= o.p1) goto epilogue;
+  {
+    GemmProd pr = prods[p];
+    gload(pr, 0);
+    lstore(0);
+    __syncthreads();
+    int buf = 0;
+    while (true) {
+      int np = p, nk0 = k0 + KT;
+      if (nk0 >= pr.k) {
+        np = p + 1;
+        nk0 = 0;
+      }
+      const bool has_next = np < o.p1;
+      GemmProd pn = pr;
+      if (has_next) {
+        if (np != p) pn = prods[np];
+        gload(pn, nk0);
+      }
+#pragma unroll
+      for (int ks = 0; ks < KT / 4; ++ks) {
+        const int kr = ks * 4 + (lane >> 4);
+        double af[FM], bf[FN];
+#pragma unroll
+        for (int i = 0; i < FM; ++i) af[i] = As[buf][kr][(wm * FM + i) * 16 + (lane & 15)];
+#pragma unroll
+        for (int j = 0; j < FN; ++j) bf[j] = Bs[buf][kr][(wn * FN + j) * 16 + (lane & 15)];
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+      }
+      if (!has_next) break;
+      lstore(buf ^ 1);
+      __syncthreads();
+      buf ^= 1;
+      p = np;
+      k0 = nk0;
+      pr = pn;
+    }
+  }
+epilogue:
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const int col = n0 + (wn * FN + j) * 16 + (lane & 15);
+      const int rowb = m0 + (wm * FM + i) * 16 + (lane >> 4);
+      if (col >= n) continue;
+      double* cp = o.c + (int64_t)col * m;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = rowb + 4 * v;
+        if (row < m) {
+          double x = alpha * acc[i][j][v];
+          if (o.acc) x = x + cp[row];
+          cp[row] = x;
+        }
+      }
+    }
+}
+
 // Wave-per-tile variant for small blocks: each wave owns one output tile of
 // (16*FM) x (16*FN), loads its MFMA operand fragments straight from global
 // memory (L1/L2 resident for atom-sized blocks) and prefetches the next k-step
@@ -238,6 +373,13 @@ struct Cfg {
         NW                                                                                          \
   }
 
+#define SA_PCFG(NAME, WM, WN, FM, FN, KT)                                                            \
+  Cfg {                                                                                              \
+    NAME, 16 * FM * WM, 16 * FN * WN, 64 * WM * WN,                                                  \
+        {k_gemm_fp64_pipe<WM, WN, FM, FN, KT, false, false>, k_gemm_fp64_pipe<WM, WN, FM, FN, KT, false, true>, \
+         k_gemm_fp64_pipe<WM, WN, FM, FN, KT, true, false>, k_gemm_fp64_pipe<WM, WN, FM, FN, KT, true, true>}  \
+  }
+
 static const Cfg kCfgs[] = {
     SA_CFG("64x64_w4_k16", 2, 2, 2, 2, 16),   // 0: uniform 64 blocks
     SA_CFG("128x128_w4_k16", 2, 2, 4, 4, 16), // 1: uniform 128 blocks
@@ -253,6 +395,10 @@ static const Cfg kCfgs[] = {
     SA_WCFG("wave32x32_n2", 2, 2, 2),         // 11
     SA_WCFG("wave64x32_n2", 4, 2, 2),         // 12
     SA_WCFG("wave64x64_n1", 4, 4, 1),         // 13
+    SA_PCFG("pipe128x128_k16", 2, 2, 4, 4, 16), // 14: pipelined, large blocks
+    SA_PCFG("pipe128x64_k16", 2, 2, 4, 2, 16),  // 15
+    SA_PCFG("pipe64x64_k16", 2, 2, 2, 2, 16),   // 16
+    SA_PCFG("pipe64x128_k16", 2, 2, 2, 4, 16),  // 17
 };
 static constexpr int kNumCfgs = (int)(sizeof(kCfgs) / sizeof(kCfgs[0]));
 
@@ -341,7 +487,7 @@ static int autotune(sa_context* c, const GemmBatch& b) {
   for (int cfg = 0; cfg < kNumCfgs; ++cfg) {
     const Cfg& k = kCfgs[cfg];
     if (cfg != best && ((int64_t)k.tm * k.tn > 4LL * pm * pn || k.tm > 2 * pm + 16 || k.tn > 2 * pn + 16)) continue;
-    if (cfg != best && (int64_t)k.tm * k.tn * 16 < (int64_t)pm * pn) continue;  // tiles far smaller than blocks
+    if (cfg != best && (int64_t)k.tm * k.tn * 16 < std::min<int64_t>((int64_t)pm * pn, 128 * 128)) continue;  // tiny tiles
     launch_cfg(c, bs, cfg);  // warm-up (item lists, caches)
     float t = 1e30f;
     for (int rep = 0; rep < 2; ++rep) {

@@ -53,7 +53,7 @@ def test_alpha_beta_existing_c(beta):
     check_mm(oC, dC, oA, dA, 0, oB, dB, 0, 1e-4, -1.25, beta)
 
 
-@pytest.mark.parametrize("tile", list(range(14)))
+@pytest.mark.parametrize("tile", list(range(18)))
 def test_every_tile_config(tile):
     rng = np.random.default_rng(11)
     w = synth.small_uneven(n=500, fill=0.3, seed=4)
@@ -64,6 +64,24 @@ def test_every_tile_config(tile):
     oC, dC = empty_pair_blocked(w.rsz, w.csz)
     n, err = check_mm(oC, dC, oA, dA, 0, oB, dB, 0, 1e-10, 1.0, 0.0, tile=tile)
     assert n > 100
+
+
+@pytest.mark.parametrize("tile", [14, 15, 16, 17])
+@pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
+def test_pipelined_tiles_transposes(tile, ta, tb):
+    """Large ragged blocks (150, with 7/3/149-wide edges) span several pipeline
+    tiles and k-stages, including partial ones, for every transpose variant."""
+    bs = 150
+    rng = np.random.default_rng(300 + tile + 2 * ta + tb)
+    ni, nk, nj = 2 * bs + 7, 2 * bs + 3, 3 * bs - 1
+    a_shape = (nk, ni) if ta else (ni, nk)
+    b_shape = (nj, nk) if tb else (nk, nj)
+    a = rand_dense(rng, synth.uniform_sizes(a_shape[0], bs), synth.uniform_sizes(a_shape[1], bs), 0.8, 0.2)
+    b = rand_dense(rng, synth.uniform_sizes(b_shape[0], bs), synth.uniform_sizes(b_shape[1], bs), 0.8, 0.2)
+    oA, dA = pair_uniform(a, bs, bs, 1e-12)
+    oB, dB = pair_uniform(b, bs, bs, 1e-12)
+    oC, dC = empty_pair_uniform(ni, nj, bs, bs)
+    check_mm(oC, dC, oA, dA, ta, oB, dB, tb, 1e-6, 0.75, 0.0, tile=tile)
 
 
 def test_binding_threshold_screens_tasks():
