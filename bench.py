@@ -90,7 +90,8 @@ def cpu_baseline_rik(w, X, budget_s=15.0):
 
 
 def bench_rik(args):
-    w = synth.rik_config()
+    w = synth.rik_config(n_occ=args.n_occ) if args.n_occ == 200 else \
+        synth.rik_config(n_occ=args.n_occ, name=f"ri_k_step_n_occ{args.n_occ}")
     ctx = sa.Context(0)
     bcs = w.b_col_sizes
     # set-up: metric, X = M^{-1/2} (Newton-Schulz on the device), timed separately
@@ -140,6 +141,8 @@ def bench_rik(args):
     log(f"FP64 step: {ms64:.1f} ms")
     g64_ms, g64_n = ctx.profile("gemm_fp64")
     cfg64 = ctx.gemm_config()[1]
+    st64 = {k: round(ctx.profile(k)[0] / args.steps, 3) for k in ("rik_transform", "rik_k")}
+    ctx.profile_reset()
 
     for _ in range(args.warmup):
         stepmix()
@@ -150,6 +153,7 @@ def bench_rik(args):
     gsl_ms, _ = ctx.profile("ozaki_slice")
     scr_ms, _ = ctx.profile("screen")
     nrm_ms, _ = ctx.profile("norms")
+    stmix = {k: round(ctx.profile(k)[0] / args.steps, 3) for k in ("rik_transform", "rik_k")}
     ctx.set_profiling(False)
     Kmix = dK.to_dense()
     err_k = rel(Kmix, K64)
@@ -182,6 +186,7 @@ def bench_rik(args):
                  "transform_tasks": int(len(t_tr)), "k_tasks": int(len(t_k)),
                  "algorithmic_flop": f_tr + f_k, "dense_equiv_flop": dense_fl,
                  "fp64_gemm_tflops": round(fp64_achieved, 3), "fp64_gemm_tile": cfg64,
+                 "stage_ms_fp64": st64, "stage_ms_split_precision": stmix,
                  "split_ms_per_step": {"gemm_ozaki": round(oz_per_step_ms, 3),
                                        "slicing": round(gsl_ms / args.steps, 3),
                                        "screening": round(scr_ms / args.steps, 3),
@@ -301,9 +306,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="rik", choices=["rik", "config1", "config2", "config3", "config5"])
     ap.add_argument("--tile", type=int, default=-1)
-    ap.add_argument("--slices-transform", type=int, default=5)
+    ap.add_argument("--slices-transform", type=int, default=4)
     ap.add_argument("--slices-k", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--n-occ", type=int, default=200, help="RI-K occupied orbitals (200 = the headline workload)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1", "n_gpus": args.gpus}))

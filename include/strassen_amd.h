@@ -47,9 +47,10 @@ typedef struct sa_bsm* sa_bsm_t;
 #define SA_SCHED_TREE 2      /* caller-supplied decision tree bits */
 typedef struct {
   int32_t mode;        /* SA_MODE_* */
-  int32_t slices;      /* Ozaki: number of slices per operand (2..6) */
+  int32_t slices;      /* Ozaki: slices per operand (FP16: 3..5, BF16: 4..6) */
   int32_t slice_type;  /* SA_SLICE_* */
-  int32_t tile;        /* GEMM tile config id, -1 = autotuned choice */
+  int32_t tile;        /* FP64: GEMM tile config id (sa_ctx_gemm_config), -1 = autotuned;
+                          Ozaki: kernel table id, ignored unless it runs this slice count/type */
   int32_t scheduler;   /* SA_SCHED_* (hybrid Strassen only) */
   int32_t reserved;
   int64_t cutoff;      /* min_size_for_strassen (main.cpp:9), elements */

@@ -24,11 +24,6 @@ namespace sa {
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
-__device__ inline int64_t xcd_remap(int64_t bid, int64_t n) {
-  const int64_t q = n / 8, r = n % 8;
-  const int64_t x = bid % 8, t = bid / 8;
-  return x * q + (x < r ? x : r) + t;
-}
 
 template <int WM, int WN, int FM, int FN, int KT, bool TA, bool TB>
 __global__ void __launch_bounds__(64 * WM * WN)

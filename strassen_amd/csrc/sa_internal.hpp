@@ -80,6 +80,15 @@ struct DevBuf {
 
 __host__ __device__ inline int64_t ceil_div(int64_t a, int64_t b) { return a == 0 ? 0 : 1 + (a - 1) / b; }
 
+// Work-item index of workgroup `bid` such that consecutive items run on the
+// same XCD: workgroups are dispatched round-robin over the 8 XCDs, so item
+// ranges of about n/8 land on one XCD and share its L2.
+__device__ inline int64_t xcd_remap(int64_t bid, int64_t n) {
+  const int64_t q = n / 8, r = n % 8;
+  const int64_t x = bid % 8, t = bid / 8;
+  return x * q + (x < r ? x : r) + t;
+}
+
 // --------------------------------------------------------------- profiling --
 struct ProfAcc {
   double ms = 0.0;

@@ -5,19 +5,29 @@
 // evaluated with low-precision matrix cores:
 //  * slicing (once per operand per call, HBM-bound): every op(A) row / op(B)
 //    column v of a block gets a power-of-two scale 2^e_v > max|x_v| and the
-//    integer n = rint(x 2^(beta*S - e_v)) (|n| <= 2^(beta*S)) is cut into S
-//    balanced beta-bit digits d_1..d_S (d_1 most significant), stored as FP16
-//    (beta = 9) or BF16 (beta = 7) values, which hold them exactly;
-//  * product: x_A x_B = sum_{p,q} d_p d_q 2^(e_r + f_c - beta (p+q)); pairs
-//    with p + q <= S + 1 are kept (the error-free-transformation split of
+//    integer n = rint(x 2^(beta*S - TS - e_v)) (|n| <= 2^(beta*S - TS)) is cut
+//    into S balanced beta-bit digits d_0..d_{S-1} (d_0 most significant,
+//    |d_0| <= 2^(beta - TS), |d_p| <= 2^(beta - 1)), stored as FP16 (beta = 10,
+//    TS = 2) or BF16 (beta = 8, TS = 1) values, which hold them exactly;
+//  * product: x_A x_B = sum_{p,q} d_p d_q 2^(e_r + f_c + 2 TS - beta (p+q+2));
+//    pairs with p + q <= S - 1 are kept (the error-free-transformation split of
 //    Ozaki, Ogita, Oishi & Rump, Numer. Algorithms 59 (2012), in the integer
 //    digit form of Ootomo & Yokota 2022 / Uchino et al. 2024);
-//  * every digit product sum over one MFMA k-step (32) is an integer below
-//    2^24, so the FP32 accumulation inside v_mfma_f32_16x16x32_{f16,bf16} is
-//    exact; after each k-step the per-weight FP32 sums are scaled by
-//    2^(e_r + f_c - beta w) and added into FP64 accumulators.
-// Truncation error per product is about 2^(-beta*S) relative to the row/column
-// maxima of the two blocks (S = 4 FP16 slices: 2^-36).
+//  * digit products of one weight w = p + q accumulate in FP32 inside
+//    v_mfma_f32_16x16x32_{f16,bf16}; the sums of the weights that carry the
+//    leading bits stay exact (integers below 2^24) between two folds into the
+//    FP64 accumulators: FP16 weight 0 for 8 k-steps of 32 (256^2*32*8 = 2^24),
+//    weight 1 for 2 (2*256*512*32*2 = 2^24); BF16 weights 0..2 for 8.  Later
+//    weights are folded on the same schedule and may round in FP32, which
+//    costs at most 2^-(2 beta + 21) of the leading term per fold.
+// Truncation error per product is about 2^-(beta*S - TS) relative to the
+// row/column maxima of the two blocks (4 FP16 slices: 2^-38).
+//
+// Slice layout: for each block, S slices of [vb][ks][lane][8] fragments, one
+// fragment = 16 vectors x 32 k = the operand of one 16x16x32 MFMA (1 KB):
+// lane l holds vector 16 vb + (l & 15) at k = 32 ks + 8 (l >> 4) + t.  Vectors
+// are padded to a multiple of 16 and k to a multiple of 32 with zeros, so every
+// fragment load is one contiguous, coalesced 1 KB read.
 #include <algorithm>
 #include <cmath>
 
@@ -31,15 +41,23 @@ typedef float flt4 __attribute__((ext_vector_type(4)));
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
 constexpr int kOzK = 32;  // MFMA k of the 16x16x32 low-precision instructions
+__host__ __device__ constexpr int oz_beta(bool bf) { return bf ? 8 : 10; }
+__host__ __device__ constexpr int oz_tshift(bool bf) { return bf ? 1 : 2; }
+__host__ __device__ constexpr double pow2i(int e) {
+  double r = 1.0;
+  for (int i = 0; i < e; ++i) r *= 2.0;
+  for (int i = 0; i > e; --i) r *= 0.5;
+  return r;
+}
 
 struct OzProd {
-  const uint16_t* a;  // S slices of op(A) block, row-major: [p][row][kp]
-  const uint16_t* b;  // S slices of op(B) block, column-major: [p][col][kp]
-  const int32_t* ea;  // row exponents
-  const int32_t* eb;  // column exponents
-  int32_t kp;         // k padded to kOzK
-  int32_t ma;         // rows of the op(A) block (slice stride)
-  int32_t nb;         // columns of the op(B) block (slice stride)
+  const uint16_t* a;  // S slices of the op(A) block (fragment layout)
+  const uint16_t* b;  // S slices of the op(B) block
+  const int32_t* ea;  // row exponents (padded to 16)
+  const int32_t* eb;  // column exponents (padded to 16)
+  int32_t nks;        // k-steps of 32
+  int32_t nvba;       // 16-row blocks of the op(A) block
+  int32_t nvbb;       // 16-column blocks of the op(B) block
   int32_t pad;
 };
 
@@ -74,45 +92,52 @@ __global__ void k_slice_need(SliceArgs s, int S, int64_t* __restrict__ need_sl, 
   int64_t sv, sk;
   slice_geom(s, b, nv, kd, sv, sk);
   const bool pres = s.off[b] >= 0;
-  const int64_t kp = (int64_t)ceil_div(kd, kOzK) * kOzK;
-  need_sl[b] = pres ? (int64_t)S * nv * kp : 0;
-  need_ex[b] = pres ? nv : 0;
+  const int64_t kp = ceil_div(kd, kOzK) * kOzK;
+  const int64_t nvp = ceil_div(nv, 16) * 16;
+  need_sl[b] = pres ? (int64_t)S * nvp * kp : 0;
+  need_ex[b] = pres ? nvp : 0;
 }
 
 template <bool BF>
-__global__ void __launch_bounds__(256) k_slice(SliceArgs s, int S, int beta, const int64_t* __restrict__ sl_off,
+__global__ void __launch_bounds__(256) k_slice(SliceArgs s, int S, const int64_t* __restrict__ sl_off,
                                                const int64_t* __restrict__ ex_off, uint16_t* __restrict__ slices,
                                                int32_t* __restrict__ exps) {
+  constexpr int BETA = oz_beta(BF), TS = oz_tshift(BF);
   const int64_t b = blockIdx.x;
   const int64_t o = s.off[b];
   if (o < 0) return;
   int nv, kd;
   int64_t sv, sk;
   slice_geom(s, b, nv, kd, sv, sk);
-  const int kp = (int)ceil_div(kd, kOzK) * kOzK;
+  const int nks = (int)ceil_div(kd, kOzK), kp = nks * kOzK;
+  const int nvb = (int)ceil_div(nv, 16), nvp = nvb * 16;
+  const int64_t pstride = (int64_t)nvp * kp;
   const double* x = s.arena + o;
   uint16_t* out = slices + sl_off[b];
   int32_t* ex = exps + ex_off[b];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int64_t mask = ((int64_t)1 << beta) - 1, half = (int64_t)1 << (beta - 1);
-  for (int v = w; v < nv; v += 4) {
+  const int64_t mask = ((int64_t)1 << BETA) - 1, half = (int64_t)1 << (BETA - 1);
+  for (int v = w; v < nvp; v += 4) {
     double mx = 0.0;
-    for (int kk = lane; kk < kd; kk += 64) mx = fmax(mx, fabs(x[v * sv + kk * sk]));
+    if (v < nv)
+      for (int kk = lane; kk < kd; kk += 64) mx = fmax(mx, fabs(x[v * sv + kk * sk]));
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
     int e = 0;
     if (mx > 0.0) (void)frexp(mx, &e);  // mx < 2^e
     if (lane == 0) ex[v] = e;
+    const int64_t vbase = (int64_t)(v >> 4) * nks * 512 + (v & 15) * 8;
     for (int kk = lane; kk < kp; kk += 64) {
-      const double a = kk < kd ? x[v * sv + kk * sk] : 0.0;
-      int64_t n = mx > 0.0 ? (int64_t)rint(ldexp(a, beta * S - e)) : 0;
+      const double a = (v < nv && kk < kd) ? x[v * sv + kk * sk] : 0.0;
+      int64_t n = mx > 0.0 ? (int64_t)rint(ldexp(a, BETA * S - TS - e)) : 0;
+      const int64_t at = vbase + (int64_t)(kk >> 5) * 512 + ((kk & 31) >> 3) * 128 + (kk & 7);
       // balanced digits, least significant first
       for (int p = S - 1; p >= 0; --p) {
         int64_t d;
         if (p > 0) {
           d = n & mask;
-          if (d >= half) d -= (int64_t)1 << beta;
-          n = (n - d) >> beta;
+          if (d >= half) d -= (int64_t)1 << BETA;
+          n = (n - d) >> BETA;
         } else {
           d = n;
         }
@@ -124,7 +149,7 @@ __global__ void __launch_bounds__(256) k_slice(SliceArgs s, int S, int beta, con
           const _Float16 h = (_Float16)(float)d;
           bits = __builtin_bit_cast(uint16_t, h);
         }
-        out[(int64_t)p * nv * kp + (int64_t)v * kp + kk] = bits;
+        out[(int64_t)p * pstride + at] = bits;
       }
     }
   }
@@ -193,110 +218,182 @@ __global__ void k_oz_records(OzTables t, GemmOut* __restrict__ outs, OzProd* __r
     pr.b = t.bsl + t.bsl_off[bi];
     pr.ea = t.aex + t.aex_off[ai];
     pr.eb = t.bex + t.bex_off[bi];
-    pr.kp = (int32_t)ceil_div(t.ksz[kb], kOzK) * kOzK;
-    pr.ma = o.m;
-    pr.nb = o.n;
+    pr.nks = (int32_t)ceil_div(t.ksz[kb], kOzK);
+    pr.nvba = (int32_t)ceil_div(o.m, 16);
+    pr.nvbb = (int32_t)ceil_div(o.n, 16);
     pr.pad = 0;
     prods[p] = pr;
   }
 }
 
 // ----------------------------------------------------------------- GEMM --
-// Wave-per-tile: each wave owns a (16*FM) x (16*FN) output tile.  Operand
-// fragments (8 consecutive k per lane) are 16-byte loads from the slices.
-template <int S, int BETA, bool BF, int FM, int FN, int NW>
-__global__ void __launch_bounds__(64 * NW)
-    k_gemm_ozaki(const GemmOut* __restrict__ outs, const OzProd* __restrict__ prods, const GemmItem* __restrict__ items,
-                 int64_t nitems, double alpha) {
-  constexpr int NWt = S;  // weights w = 2 .. S+1
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int64_t it = (int64_t)blockIdx.x * NW + w;
-  if (it >= nitems) return;
+// One workgroup of 4 waves (2 x 2) owns a (32 FM) x (32 FN) output tile; each
+// wave owns (16 FM) x (16 FN).  A stage is one k-step of 32 for all S slices of
+// the tile's op(A) rows and op(B) columns, i.e. S (2 FM + 2 FN) fragments of
+// 1 KB, double-buffered in LDS.  The stream of stages runs over the products
+// of the output; the global loads of stage s+1 are in flight while the MFMAs of
+// stage s run, with one barrier per stage.  Fragments of 16-row blocks beyond
+// the output block are skipped (wave-uniform).
+template <int S, bool BF, int FM, int FN>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+    k_gemm_ozaki(const GemmOut* __restrict__ outs, const OzProd* __restrict__ prods,
+                 const GemmItem* __restrict__ items, int64_t nitems, double alpha) {
+  constexpr int BETA = oz_beta(BF), TS = oz_tshift(BF);
+  constexpr int RA = 2 * FM, RB = 2 * FN;    // 16-blocks of rows / columns per tile
+  constexpr int NCA = S * RA, NCB = S * RB;  // fragments per stage
+  constexpr int LA = (NCA + 3) / 4, LB = (NCB + 3) / 4;
+  constexpr int FREQ_LO = 1, FREQ_HI = (BF || S < 3) ? 1 : 3;  // weights folded every 2 k-steps (FP16)
+  __shared__ uint4 As[2][NCA][64];
+  __shared__ uint4 Bs[2][NCB][64];
+
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int wm = wid & 1, wn = wid >> 1;
+  const int li = lane & 15, lg = lane >> 4;
+  const int64_t it = xcd_remap(blockIdx.x, nitems);
   const GemmItem item = items ? items[it] : GemmItem{(int32_t)it, 0, 0};
   const GemmOut o = outs[item.out];
   const int m = o.m, n = o.n;
-  const int m0 = item.tm * 16 * FM, n0 = item.tn * 16 * FN;
-  const int li = lane & 15, lg = lane >> 4;
+  const int nrb = (int)ceil_div(m, 16), ncb = (int)ceil_div(n, 16);
+  const int rb0 = item.tm * RA, cb0 = item.tn * RB;
+  bool act_i[FM], act_j[FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i) act_i[i] = rb0 + wm * FM + i < nrb;
+#pragma unroll
+  for (int j = 0; j < FN; ++j) act_j[j] = cb0 + wn * FN + j < ncb;
 
   dbl4 acc[FM][FN];
+  flt4 s32[S][FM][FN];
 #pragma unroll
   for (int i = 0; i < FM; ++i)
 #pragma unroll
-    for (int j = 0; j < FN; ++j) acc[i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < FN; ++j) {
+      acc[i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int w = 0; w < S; ++w) s32[w][i][j] = flt4{0.f, 0.f, 0.f, 0.f};
+    }
+  int er[FM][4], ec[FN];
+  uint4 ra[LA], rbf[LB];
 
-  for (int p = o.p0; p < o.p1; ++p) {
-    const OzProd pr = prods[p];
-    // exponents of the rows (D layout: row 4*lg + v) and columns this lane folds
-    int er[FM][4], ec[FN];
+  auto gload = [&](const OzProd& pr, int ks) {
+    const uint4* a4 = reinterpret_cast<const uint4*>(pr.a);
+    const uint4* b4 = reinterpret_cast<const uint4*>(pr.b);
+    const int64_t psa = (int64_t)pr.nvba * pr.nks * 64, psb = (int64_t)pr.nvbb * pr.nks * 64;
+#pragma unroll
+    for (int r = 0; r < LA; ++r) {
+      const int c = wid + 4 * r;
+      const int p = c / RA, vb = rb0 + c % RA;
+      ra[r] = (c < NCA && vb < nrb) ? a4[p * psa + ((int64_t)vb * pr.nks + ks) * 64 + lane] : make_uint4(0, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < LB; ++r) {
+      const int c = wid + 4 * r;
+      const int p = c / RB, vb = cb0 + c % RB;
+      rbf[r] = (c < NCB && vb < ncb) ? b4[p * psb + ((int64_t)vb * pr.nks + ks) * 64 + lane] : make_uint4(0, 0, 0, 0);
+    }
+  };
+  auto lstore = [&](int buf) {
+#pragma unroll
+    for (int r = 0; r < LA; ++r) {
+      const int c = wid + 4 * r;
+      if (c < NCA) As[buf][c][lane] = ra[r];
+    }
+#pragma unroll
+    for (int r = 0; r < LB; ++r) {
+      const int c = wid + 4 * r;
+      if (c < NCB) Bs[buf][c][lane] = rbf[r];
+    }
+  };
+  auto load_exps = [&](const OzProd& pr) {
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int r = m0 + i * 16 + 4 * lg + v;
-        er[i][v] = r < m ? pr.ea[r] : 0;
-      }
+      for (int v = 0; v < 4; ++v) er[i][v] = act_i[i] ? pr.ea[(rb0 + wm * FM + i) * 16 + 4 * lg + v] : 0;
 #pragma unroll
-    for (int j = 0; j < FN; ++j) {
-      const int cc = n0 + j * 16 + li;
-      ec[j] = cc < n ? pr.eb[cc] : 0;
-    }
-    const int64_t sa_ = (int64_t)pr.ma * pr.kp, sb_ = (int64_t)pr.nb * pr.kp;
-    for (int k0 = 0; k0 < pr.kp; k0 += kOzK) {
-      uint4 af[S][FM], bfr[S][FN];
+    for (int j = 0; j < FN; ++j) ec[j] = act_j[j] ? pr.eb[(cb0 + wn * FN + j) * 16 + li] : 0;
+  };
+  auto compute = [&](int buf) {
+    uint4 bq[S][FN];
 #pragma unroll
-      for (int q = 0; q < S; ++q) {
+    for (int q = 0; q < S; ++q)
 #pragma unroll
-        for (int i = 0; i < FM; ++i) {
-          const int r = m0 + i * 16 + li;
-          af[q][i] = r < m ? *reinterpret_cast<const uint4*>(pr.a + q * sa_ + (int64_t)r * pr.kp + k0 + 8 * lg)
-                           : make_uint4(0, 0, 0, 0);
-        }
+      for (int j = 0; j < FN; ++j) bq[q][j] = Bs[buf][q * RB + wn * FN + j][lane];
 #pragma unroll
-        for (int j = 0; j < FN; ++j) {
-          const int cc = n0 + j * 16 + li;
-          bfr[q][j] = cc < n ? *reinterpret_cast<const uint4*>(pr.b + q * sb_ + (int64_t)cc * pr.kp + k0 + 8 * lg)
-                             : make_uint4(0, 0, 0, 0);
-        }
-      }
-      flt4 s32[NWt][FM][FN];
+    for (int pa = 0; pa < S; ++pa) {
+      uint4 aq[FM];
 #pragma unroll
-      for (int x = 0; x < NWt; ++x)
+      for (int i = 0; i < FM; ++i) aq[i] = As[buf][pa * RA + wm * FM + i][lane];
+#pragma unroll
+      for (int pb = 0; pb < S - pa; ++pb)
 #pragma unroll
         for (int i = 0; i < FM; ++i)
 #pragma unroll
-          for (int j = 0; j < FN; ++j) s32[x][i][j] = flt4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int pa = 0; pa < S; ++pa)
-#pragma unroll
-        for (int pb = 0; pb < S; ++pb) {
-          if (pa + pb > S - 1) continue;  // keep p + q <= S + 1 (1-based)
-#pragma unroll
-          for (int i = 0; i < FM; ++i)
-#pragma unroll
-            for (int j = 0; j < FN; ++j) {
-              if (BF) {
-                s32[pa + pb][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                    __builtin_bit_cast(bf16x8, af[pa][i]), __builtin_bit_cast(bf16x8, bfr[pb][j]), s32[pa + pb][i][j], 0,
-                    0, 0);
-              } else {
-                s32[pa + pb][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
-                    __builtin_bit_cast(half8, af[pa][i]), __builtin_bit_cast(half8, bfr[pb][j]), s32[pa + pb][i][j], 0,
-                    0, 0);
-              }
-            }
-        }
-      // fold: weight index x corresponds to p + q = x + 2 (1-based)
-#pragma unroll
-      for (int i = 0; i < FM; ++i)
-#pragma unroll
-        for (int j = 0; j < FN; ++j)
-#pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            double t = 0.0;
-#pragma unroll
-            for (int x = 0; x < NWt; ++x) t += ldexp((double)s32[x][i][j][v], -BETA * (x + 2));
-            acc[i][j][v] += ldexp(t, er[i][v] + ec[j]);
+          for (int j = 0; j < FN; ++j) {
+            if (!(act_i[i] && act_j[j])) continue;
+            if (BF)
+              s32[pa + pb][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                  __builtin_bit_cast(bf16x8, aq[i]), __builtin_bit_cast(bf16x8, bq[pb][j]), s32[pa + pb][i][j], 0, 0, 0);
+            else
+              s32[pa + pb][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+                  __builtin_bit_cast(half8, aq[i]), __builtin_bit_cast(half8, bq[pb][j]), s32[pa + pb][i][j], 0, 0, 0);
           }
+    }
+  };
+  // acc += sum_{w in [W0, W1)} s_w 2^(e_r + f_c + 2 TS - beta (w + 2)); s_w = 0
+  auto fold = [&](auto w0c, auto w1c) {
+    constexpr int W0 = decltype(w0c)::value, W1 = decltype(w1c)::value;
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        if (!(act_i[i] && act_j[j])) continue;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          double t = 0.0;
+#pragma unroll
+          for (int w = W0; w < W1; ++w) t = fma((double)s32[w][i][j][v], pow2i(-BETA * w), t);
+          acc[i][j][v] += ldexp(t, er[i][v] + ec[j] + 2 * TS - 2 * BETA);
+        }
+#pragma unroll
+        for (int w = W0; w < W1; ++w) s32[w][i][j] = flt4{0.f, 0.f, 0.f, 0.f};
+      }
+  };
+
+  int p = o.p0;
+  if (p < o.p1) {
+    OzProd pr = prods[p];
+    int ks = 0;
+    load_exps(pr);
+    gload(pr, 0);
+    lstore(0);
+    __syncthreads();
+    int buf = 0;
+    while (true) {
+      int np = p, nks = ks + 1;
+      if (nks >= pr.nks) {
+        np = p + 1;
+        nks = 0;
+      }
+      const bool has_next = np < o.p1;
+      OzProd pn = pr;
+      if (has_next) {
+        if (np != p) pn = prods[np];
+        gload(pn, nks);
+      }
+      compute(buf);
+      const int done = ks + 1;
+      if (np != p || (done & 7) == 0) {
+        fold(std::integral_constant<int, 0>{}, std::integral_constant<int, S>{});
+      } else if (FREQ_HI > FREQ_LO && (done & 1) == 0) {
+        fold(std::integral_constant<int, FREQ_LO>{}, std::integral_constant<int, FREQ_HI>{});
+      }
+      if (!has_next) break;
+      if (np != p) load_exps(pn);
+      lstore(buf ^ 1);
+      __syncthreads();
+      buf ^= 1;
+      p = np;
+      ks = nks;
+      pr = pn;
     }
   }
 
@@ -304,12 +401,13 @@ __global__ void __launch_bounds__(64 * NW)
   for (int i = 0; i < FM; ++i)
 #pragma unroll
     for (int j = 0; j < FN; ++j) {
-      const int col = n0 + j * 16 + li;
+      if (!(act_i[i] && act_j[j])) continue;
+      const int col = (cb0 + wn * FN + j) * 16 + li;
       if (col >= n) continue;
       double* cp = o.c + (int64_t)col * m;
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int row = m0 + i * 16 + 4 * lg + v;
+        const int row = (rb0 + wm * FM + i) * 16 + 4 * lg + v;
         if (row < m) {
           double x = alpha * acc[i][j][v];
           if (o.acc) x = x + cp[row];
@@ -322,19 +420,19 @@ __global__ void __launch_bounds__(64 * NW)
 using OzFn = void (*)(const GemmOut*, const OzProd*, const GemmItem*, int64_t, double);
 
 struct OzCfg {
-  int S, beta;
+  int S;
   bool bf;
-  int fm, fn, nw;
+  int fm, fn;
   OzFn fn_;
 };
 
-#define SA_OZ(S, BETA, BF, FM, FN, NW) OzCfg{S, BETA, BF, FM, FN, NW, k_gemm_ozaki<S, BETA, BF, FM, FN, NW>}
+#define SA_OZ(S, BF, FM, FN) OzCfg{S, BF, FM, FN, k_gemm_ozaki<S, BF, FM, FN>}
 
 static const OzCfg kOzCfgs[] = {
-    SA_OZ(3, 9, false, 2, 2, 4), SA_OZ(4, 9, false, 2, 2, 4), SA_OZ(5, 9, false, 2, 2, 4),
-    SA_OZ(3, 9, false, 1, 1, 4), SA_OZ(4, 9, false, 1, 1, 4), SA_OZ(5, 9, false, 1, 1, 4),
-    SA_OZ(4, 7, true, 2, 2, 4),  SA_OZ(5, 7, true, 2, 2, 4),  SA_OZ(6, 7, true, 2, 2, 4),
-    SA_OZ(4, 7, true, 1, 1, 4),  SA_OZ(5, 7, true, 1, 1, 4),  SA_OZ(6, 7, true, 1, 1, 4),
+    SA_OZ(3, false, 2, 2), SA_OZ(4, false, 2, 2), SA_OZ(5, false, 2, 2),
+    SA_OZ(3, false, 1, 1), SA_OZ(4, false, 1, 1), SA_OZ(5, false, 1, 1),
+    SA_OZ(4, true, 2, 2),  SA_OZ(5, true, 2, 2),  SA_OZ(6, true, 2, 2),
+    SA_OZ(4, true, 1, 1),  SA_OZ(5, true, 1, 1),  SA_OZ(6, true, 1, 1),
 };
 
 __global__ void k_count_tiles_oz(const GemmOut* __restrict__ outs, int64_t nouts, int TM, int TN,
@@ -362,8 +460,7 @@ struct Sliced {
   const int64_t* ex_off;
 };
 
-static Sliced slice_operand(sa_context* c, sa_bsm* X, bool role_b, bool trans, int S, int beta, bool bf,
-                            const char* tag) {
+static Sliced slice_operand(sa_context* c, sa_bsm* X, bool role_b, bool trans, int S, bool bf, const char* tag) {
   const int64_t nb = X->nb();
   std::string t(tag);
   DevBuf& bso = c->buf((t + "_sloff").c_str());
@@ -389,10 +486,10 @@ static Sliced slice_operand(sa_context* c, sa_bsm* X, bool role_b, bool trans, i
     ProfScope ps(c, "ozaki_slice");
     if (nb) {
       if (bf)
-        hipLaunchKernelGGL(k_slice<true>, dim3((unsigned)nb), dim3(256), 0, c->stream, a, S, beta,
+        hipLaunchKernelGGL(k_slice<true>, dim3((unsigned)nb), dim3(256), 0, c->stream, a, S,
                            (const int64_t*)bso.p, (const int64_t*)beo.p, bsl.as<uint16_t>(), bex.as<int32_t>());
       else
-        hipLaunchKernelGGL(k_slice<false>, dim3((unsigned)nb), dim3(256), 0, c->stream, a, S, beta,
+        hipLaunchKernelGGL(k_slice<false>, dim3((unsigned)nb), dim3(256), 0, c->stream, a, S,
                            (const int64_t*)bso.p, (const int64_t*)beo.p, bsl.as<uint16_t>(), bex.as<int32_t>());
     }
   }
@@ -401,7 +498,7 @@ static Sliced slice_operand(sa_context* c, sa_bsm* X, bool role_b, bool trans, i
 }
 
 static int pick_cfg(int S, bool bf, int max_m, int max_n, int tile) {
-  const int want_fm = (std::max(max_m, max_n) > 16) ? 2 : 1;
+  const int want_fm = (std::max(max_m, max_n) > 32) ? 2 : 1;
   for (int i = 0; i < (int)(sizeof(kOzCfgs) / sizeof(kOzCfgs[0])); ++i) {
     const OzCfg& k = kOzCfgs[i];
     if (k.S == S && k.bf == bf && (tile < 0 ? k.fm == want_fm : i == tile)) return i;
@@ -414,7 +511,6 @@ int64_t ozaki_multiply(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, double
   sa_context* c = C->ctx;
   const bool bf = o.slice_type == SA_SLICE_BF16;
   const int S = o.slices;
-  const int beta = bf ? 7 : 9;
   const int cfg = pick_cfg(S, bf, 64, 64, -1);
   SA_REQUIRE(cfg >= 0, "ozaki: %d %s slices not supported (FP16: 3..5, BF16: 4..6)", S, bf ? "BF16" : "FP16");
   const ScreenResult R = screen_and_allocate(C, A, ta, B, tb, thresh, nullptr, nullptr, 0, false);
@@ -422,8 +518,8 @@ int64_t ozaki_multiply(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, double
   // slice operands; share when both roles read the same vectors of one matrix
   const sa_bsm* ra = A->base ? A->base : A;
   const sa_bsm* rb = B->base ? B->base : B;
-  const Sliced sa_ = slice_operand(c, A, false, ta, S, beta, bf, "oz_a");
-  const Sliced sb_ = (ra == rb && ta != tb && A == B) ? sa_ : slice_operand(c, B, true, tb, S, beta, bf, "oz_b");
+  const Sliced sa_ = slice_operand(c, A, false, ta, S, bf, "oz_a");
+  const Sliced sb_ = (ra == rb && ta != tb && A == B) ? sa_ : slice_operand(c, B, true, tb, S, bf, "oz_b");
 
   DevBuf& bouts = c->buf("mm_outs");
   DevBuf& bprods = c->buf("oz_prods");
@@ -446,9 +542,10 @@ int64_t ozaki_multiply(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, double
   int32_t mm = 0, mn = 0;
   for (int32_t v : C->rsz) mm = std::max(mm, v);
   for (int32_t v : C->csz) mn = std::max(mn, v);
-  const int cf = pick_cfg(S, bf, mm, mn, -1);
+  int cf = o.tile >= 0 ? pick_cfg(S, bf, mm, mn, o.tile) : -1;  // a forced id that runs other slices is ignored
+  if (cf < 0) cf = pick_cfg(S, bf, mm, mn, -1);
   const OzCfg& k = kOzCfgs[cf];
-  const int TM = 16 * k.fm, TN = 16 * k.fn;
+  const int TM = 32 * k.fm, TN = 32 * k.fn;
   const GemmItem* items = nullptr;
   int64_t nitems = R.nouts;
   if (mm > TM || mn > TN) {
@@ -465,7 +562,7 @@ int64_t ozaki_multiply(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, double
   SA_REQUIRE(nitems < ((int64_t)1 << 31), "ozaki: too many work items");
   {
     ProfScope ps(c, "gemm_ozaki");
-    hipLaunchKernelGGL(k.fn_, dim3((unsigned)ceil_div(nitems, k.nw)), dim3(64 * k.nw), 0, c->stream,
+    hipLaunchKernelGGL(k.fn_, dim3((unsigned)nitems), dim3(256), 0, c->stream,
                        (const GemmOut*)bouts.as<GemmOut>(), (const OzProd*)bprods.as<OzProd>(), items, nitems, alpha);
   }
   SA_CHECK_HIP(hipGetLastError());

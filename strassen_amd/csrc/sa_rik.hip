@@ -189,12 +189,15 @@ int sa_ri_k_step(sa_bsm_t X, sa_bsm_t B, sa_bsm_t Bp, sa_bsm_t K, int32_t n_mu, 
     SA_CHECK_HIP(hipSetDevice(B->ctx->device));
     // transform B' = X B
     check_mm_geometry(Bp, X, false, B, false);
-    beta_prepass(Bp, 0.0);
-    if (ot.mode == SA_MODE_OZAKI)
-      ozaki_multiply(Bp, X, false, B, false, thresh, 1.0, ot);
-    else
-      screened_multiply(Bp, X, false, B, false, thresh, 1.0, ot.tile, nullptr, nullptr, 0, false);
-    launch_block_norms(Bp);
+    {
+      ProfScope ps(B->ctx, "rik_transform");
+      beta_prepass(Bp, 0.0);
+      if (ot.mode == SA_MODE_OZAKI)
+        ozaki_multiply(Bp, X, false, B, false, thresh, 1.0, ot);
+      else
+        screened_multiply(Bp, X, false, B, false, thresh, 1.0, ot.tile, nullptr, nullptr, 0, false);
+      launch_block_norms(Bp);
+    }
     // K-view of B': rows (P, i-chunk), columns mu
     const int32_t npb = Bp->nrb;
     std::vector<int32_t> vr((size_t)npb * n_ic), vrs((size_t)npb * n_ic);
@@ -209,6 +212,7 @@ int sa_ri_k_step(sa_bsm_t X, sa_bsm_t B, sa_bsm_t Bp, sa_bsm_t K, int32_t n_mu, 
     if (rc != SA_OK) throw Error{rc};
     try {
       check_mm_geometry(K, view, true, view, false);
+      ProfScope ps(B->ctx, "rik_k");
       beta_prepass(K, 0.0);
       if (ok.mode == SA_MODE_OZAKI)
         ozaki_multiply(K, view, true, view, false, thresh, 1.0, ok);

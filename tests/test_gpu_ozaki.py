@@ -67,6 +67,22 @@ def test_uneven_transposes(ta, tb):
     assert rel_fro(got, ref) < TOL
 
 
+@pytest.mark.parametrize("stype,slices", [(sa.SLICE_FP16, 4), (sa.SLICE_BF16, 5)])
+def test_deep_k_fold_schedule(stype, slices):
+    """Products with k = 1000 and 1024 (32 k-steps, one ragged) chain over
+    several fold intervals and product boundaries of one output block."""
+    rng = np.random.default_rng(21)
+    ksz = [1024, 1000, 1024, 37]
+    a = rng.standard_normal((80, sum(ksz))) * np.exp(rng.standard_normal((80, 1)))
+    b = rng.standard_normal((sum(ksz), 72)) * np.exp(rng.standard_normal((1, 72)))
+    oA, dA = pair_blocked(a, [80], ksz, 0.0)
+    oB, dB = pair_blocked(b, ksz, [40, 32], 0.0)
+    oC, dC = empty_pair_blocked([80], [40, 32])
+    ref, got = run(oA, dA, 0, oB, dB, 0, oC, dC, 0.0, sa.default_opts(mode=sa.MODE_OZAKI, slices=slices,
+                                                                       slice_type=stype))
+    assert rel_fro(got, ref) < TOL
+
+
 def test_integer_inputs_exact():
     """Inputs with at most 3 significant bits per row max live in the leading
     slice, so every dropped slice product is zero and the result is exact."""
