@@ -77,6 +77,13 @@ int sa_ctx_profile_reset(sa_ctx_t ctx);
 /* Tile configuration id and name used by the context's last FP64 GEMM launch
  * (chosen by the autotuner unless opts->tile forced one). */
 int sa_ctx_gemm_config(sa_ctx_t ctx, int32_t* cfg, char* name, int32_t name_len);
+/* Number of k-windows (grouped launches in ascending inner-block order) the
+ * context's last screened multiply ran in; 1 = one launch.  Windows are used
+ * when the operands far exceed the 256 MB Infinity Cache while the written C
+ * blocks fit in it; env SA_KWINDOW_MB sets the operand budget per window
+ * (default 160, 0 = never).  No counterpart in the reference, which runs
+ * per-block dgemm calls (blocksparsematrix.hpp:363-405). */
+int sa_ctx_kwindows(sa_ctx_t ctx, int32_t* nwin);
 /* Wall time of the context stream between two sa_ctx_mark calls (ms). */
 int sa_ctx_mark(sa_ctx_t ctx, int slot);
 int sa_ctx_elapsed(sa_ctx_t ctx, int slot0, int slot1, float* ms);

@@ -62,6 +62,7 @@ ABI = {
     "sa_ctx_profile": (INT, [P, C.c_char_p, P, P]),
     "sa_ctx_profile_reset": (INT, [P]),
     "sa_ctx_gemm_config": (INT, [P, P, C.c_char_p, I32]),
+    "sa_ctx_kwindows": (INT, [P, P]),
     "sa_ctx_mark": (INT, [P, INT]),
     "sa_ctx_elapsed": (INT, [P, INT, INT, P]),
     "sa_bsm_create": (INT, [P, I64, I64, I32, I32, D, P]),
@@ -182,6 +183,12 @@ class Context:
         buf = C.create_string_buffer(64)
         _check(lib().sa_ctx_gemm_config(self.h, C.byref(cfg), buf, 64))
         return cfg.value, buf.value.decode()
+
+    def kwindows(self) -> int:
+        """k-windows of the last screened multiply (1 = one grouped launch)."""
+        n = I32(0)
+        _check(lib().sa_ctx_kwindows(self.h, C.byref(n)))
+        return n.value
 
     def mark(self, slot: int):
         _check(lib().sa_ctx_mark(self.h, slot))

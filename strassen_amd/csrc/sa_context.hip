@@ -18,7 +18,11 @@ void set_error(const char* fmt, ...) {
 
 void DevBuf::reserve(size_t n) {
   if (n <= bytes) return;
-  if (p) SA_CHECK_HIP(hipFree(p));
+  if (p) {
+    // kernels queued earlier may still read the old buffer
+    SA_CHECK_HIP(hipDeviceSynchronize());
+    SA_CHECK_HIP(hipFree(p));
+  }
   p = nullptr;
   bytes = 0;
   size_t want = n < 4096 ? 4096 : n;
@@ -293,6 +297,13 @@ int sa_ctx_gemm_config(sa_ctx_t c, int32_t* cfg, char* name, int32_t name_len) {
     if (name && name_len > 0) {
       std::snprintf(name, (size_t)name_len, "%s", c->last_gemm_cfg >= 0 ? gemm_fp64_config_name(c->last_gemm_cfg) : "");
     }
+  });
+}
+
+int sa_ctx_kwindows(sa_ctx_t c, int32_t* nwin) {
+  return guard([&] {
+    SA_REQUIRE(c && nwin, "null argument");
+    *nwin = c->last_kwindows;
   });
 }
 

@@ -139,8 +139,8 @@ __global__ void __launch_bounds__(64 * WM * WN)
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const int wm = wid % WM, wn = wid / WM;
   const int64_t it = xcd_remap(blockIdx.x, nitems);
-  const GemmItem item = items ? items[it] : GemmItem{(int32_t)it, 0, 0};
-  const GemmOut o = outs[item.out];
+  const GemmItem item = items ? gload1(items + it) : GemmItem{(int32_t)it, 0, 0};
+  const GemmOut o = gload1(outs + item.out);
   const int m = o.m, n = o.n;
   const int m0 = item.tm * TM, n0 = item.tn * TN;
 
@@ -159,7 +159,7 @@ __global__ void __launch_bounds__(64 * WM * WN)
       const int i = TA ? e / KT : e % TM;
       const int kk = TA ? e % KT : e / TM;
       const int gi = m0 + i, gk = k0 + kk;
-      ra[r] = (gi < m && gk < k) ? (TA ? pr.a[gk + (int64_t)gi * k] : pr.a[gi + (int64_t)gk * m]) : 0.0;
+      ra[r] = (gi < m && gk < k) ? gload1(pr.a + (TA ? gk + (int64_t)gi * k : gi + (int64_t)gk * m)) : 0.0;
     }
 #pragma unroll
     for (int r = 0; r < LB; ++r) {
@@ -167,7 +167,7 @@ __global__ void __launch_bounds__(64 * WM * WN)
       const int j = TB ? e % TN : e / KT;
       const int kk = TB ? e / TN : e % KT;
       const int gj = n0 + j, gk = k0 + kk;
-      rb[r] = (gj < n && gk < k) ? (TB ? pr.b[gj + (int64_t)gk * n] : pr.b[gk + (int64_t)gj * k]) : 0.0;
+      rb[r] = (gj < n && gk < k) ? gload1(pr.b + (TB ? gj + (int64_t)gk * n : gk + (int64_t)gj * k)) : 0.0;
     }
   };
   auto lstore = [&](int buf) {
@@ -190,7 +190,9 @@ __global__ void __launch_bounds__(64 * WM * WN)
   int p = o.p0, k0 = 0;
   if (p >= o.p1) goto epilogue;
   {
-    GemmProd pr = prods[p];
+    GemmProd pr = gload1(prods + p);
+    // record of the following product, loaded one product ahead
+    GemmProd pf = p + 1 < o.p1 ? gload1(prods + p + 1) : pr;
     gload(pr, 0);
     lstore(0);
     __syncthreads();
@@ -204,7 +206,10 @@ __global__ void __launch_bounds__(64 * WM * WN)
       const bool has_next = np < o.p1;
       GemmProd pn = pr;
       if (has_next) {
-        if (np != p) pn = prods[np];
+        if (np != p) {
+          pn = pf;
+          if (np + 1 < o.p1) pf = gload1(prods + np + 1);
+        }
         gload(pn, nk0);
       }
 #pragma unroll

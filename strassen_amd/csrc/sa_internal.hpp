@@ -89,6 +89,39 @@ __device__ inline int64_t xcd_remap(int64_t bid, int64_t n) {
   return x * q + (x < r ? x : r) + t;
 }
 
+// Load through the global address space.  Pointers reached through records
+// are generic, so plain dereferences compile to flat loads, which also count
+// on the LDS wait counter: every wait for an LDS read would then wait for the
+// in-flight operand prefetch of the next pipeline stage as well.
+template <class T>
+__device__ inline T gload1(const T* p) {
+  static_assert(sizeof(T) % 4 == 0, "32-bit words");
+  constexpr int N = (int)(sizeof(T) / 4);
+  typedef const __attribute__((address_space(1))) unsigned* gptr;
+  const gptr q = (gptr)__builtin_assume_aligned(p, alignof(T));
+  unsigned w[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) w[i] = q[i];
+  T v;
+  __builtin_memcpy(&v, w, sizeof(T));
+  return v;
+}
+
+// Position of C block (ib, jb) in the output-record order: 4 x 4 groups of C
+// blocks are consecutive, so the tiles of neighbouring outputs, which share
+// op(A) block rows and op(B) block columns, run together on one XCD (see
+// xcd_remap) and meet in its L2.  Slots form a padded grid of
+// ceil(nib/4)*4 x ceil(njb/4)*4.
+constexpr int kOutGroup = 4;
+__host__ __device__ inline int64_t out_slot(int ib, int jb, int nib) {
+  const int64_t ngi = ceil_div(nib, kOutGroup);
+  return ((int64_t)(ib / kOutGroup) + (int64_t)(jb / kOutGroup) * ngi) * (kOutGroup * kOutGroup) + ib % kOutGroup +
+         (jb % kOutGroup) * kOutGroup;
+}
+inline int64_t out_slots(int nib, int njb) {
+  return ceil_div(nib, kOutGroup) * ceil_div(njb, kOutGroup) * kOutGroup * kOutGroup;
+}
+
 // --------------------------------------------------------------- profiling --
 struct ProfAcc {
   double ms = 0.0;
@@ -110,6 +143,7 @@ struct sa_context {
   int64_t last_ntasks = 0, last_nc = 0;
   int32_t last_nib = 0;
   int32_t last_gemm_cfg = -1;
+  int32_t last_kwindows = 1;  // k-windows of the last screened multiply
   sa::DevBuf& buf(const char* name) { return bufs[name]; }
   // profiling
   bool profiling = false;
@@ -223,13 +257,29 @@ struct ScreenResult {
   const int64_t* cnt = nullptr;
   const int64_t* seg = nullptr;
   const int64_t* need_scan = nullptr;
-  const int64_t* out_scan = nullptr;
+  const int64_t* out_scan = nullptr;  // indexed by out_slot(ib, jb, nib)
   const int32_t* kb = nullptr;
 };
 ScreenResult screen_and_allocate(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, double thresh,
                                  const uint8_t* d_tree, const int64_t* d_lvl_off, int nlev, bool c_must_exist);
 int64_t ozaki_multiply(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, double thresh, double alpha,
                        const sa_opts_t& o);
+// k-window batching.  When the operands of a multiply far exceed the
+// memory-side cache (Infinity Cache, 256 MB) while the C blocks it writes fit
+// in it, the products are split by inner block index kb into windows of
+// consecutive kb whose operand bytes fit a budget, and one grouped launch runs
+// per window, in ascending kb order: the outputs of window w carry the
+// sub-range of their product segment that falls in w (acc = 1 after an
+// output's first window), so every C block still sums its products in kb
+// order, and all tiles reading a window's operands run close together in time.
+struct KWindows {
+  int nwin = 1;
+  const GemmOut* outs = nullptr;  // device, grouped by window
+  std::vector<int64_t> off;       // nwin + 1 offsets into outs
+};
+// nwin == 1 (and outs == full records) when windows do not pay.
+KWindows plan_kwindows(sa_context* c, sa_bsm* A, bool ta, sa_bsm* B, bool tb, const ScreenResult& R,
+                       const GemmOut* outs);
 // Tile configurations of the FP64 MFMA kernel.
 int gemm_fp64_num_configs();
 const char* gemm_fp64_config_name(int cfg);

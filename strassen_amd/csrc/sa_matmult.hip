@@ -18,6 +18,8 @@
 // Mat(ni, nj, alloc) at blocksparsematrix.hpp:398-404.
 #include <cstring>
 
+#include <cstdlib>
+
 #include "sa_internal.hpp"
 
 namespace sa {
@@ -164,7 +166,7 @@ __global__ void k_build_records(MMTables t, GemmOut* __restrict__ outs, GemmProd
   o.p1 = (int32_t)(seg + cnt);
   o.acc = fresh ? 0 : 1;
   o.pad = 0;
-  outs[t.out_scan[c]] = o;
+  outs[t.out_scan[out_slot(ib, jb, t.nib)]] = o;
   for (int64_t p = seg; p < seg + cnt; ++p) {
     const int kb = t.kb[p];
     const int64_t ai = t.ta ? (int64_t)kb + (int64_t)ib * t.a_nrb : (int64_t)ib + (int64_t)kb * t.a_nrb;
@@ -186,7 +188,7 @@ __global__ void k_need_and_flags(const int64_t* __restrict__ cnt, const int64_t*
   const bool any = cnt[c] > 0;
   const int ib = (int)(c % nib), jb = (int)(c / nib);
   need[c] = (any && coff[c] < 0) ? (int64_t)crsz[ib] * ccsz[jb] : 0;
-  outflag[c] = any ? 1 : 0;
+  outflag[out_slot(ib, jb, nib)] = any ? 1 : 0;
 }
 
 // geometry of op(X) for screening / checking
@@ -244,7 +246,8 @@ ScreenResult screen_and_allocate(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool 
   bcnt.reserve(sizeof(int64_t) * (size_t)(nc + 1));
   bseg.reserve(sizeof(int64_t) * (size_t)(nc + 1));
   bneed.reserve(sizeof(int64_t) * (size_t)(nc + 1));
-  bout.reserve(sizeof(int64_t) * (size_t)(nc + 1));
+  const int64_t nslots = out_slots(nib, njb);
+  bout.reserve(sizeof(int64_t) * (size_t)(nslots + 1));
   btot.reserve(sizeof(int64_t) * 8);
   int64_t* cnt = bcnt.as<int64_t>();
   int64_t* seg = bseg.as<int64_t>();
@@ -289,12 +292,13 @@ ScreenResult screen_and_allocate(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool 
                        (int32_t*)nullptr);
   }
   SA_CHECK_HIP(hipMemcpyAsync(seg, cnt, sizeof(int64_t) * nc, hipMemcpyDeviceToDevice, c->stream));
+  fill_i64(c, outf, nslots, 0);
   hipLaunchKernelGGL(k_need_and_flags, dim3((unsigned)ceil_div(nc, 256)), dim3(256), 0, c->stream,
                      (const int64_t*)cnt, (const int64_t*)C->d_off, (const int32_t*)C->d_rsz,
                      (const int32_t*)C->d_csz, nib, nc, need, outf);
   scan_exclusive_i64_async(c, seg, nc, tot + 0);
   scan_exclusive_i64_async(c, need, nc, tot + 1);
-  scan_exclusive_i64_async(c, outf, nc, tot + 2);
+  scan_exclusive_i64_async(c, outf, nslots, tot + 2);
   int64_t* h = static_cast<int64_t*>(pinned(c, sizeof(int64_t) * 3));
   SA_CHECK_HIP(hipMemcpyAsync(h, tot, sizeof(int64_t) * 3, hipMemcpyDeviceToHost, c->stream));
   SA_CHECK_HIP(hipStreamSynchronize(c->stream));
@@ -359,8 +363,146 @@ int64_t screened_multiply(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, dou
   for (int32_t v : C->csz) mn = std::max(mn, v);
   for (int32_t v : *ga.csz) mk = std::max(mk, v);
   gb2.max_m = mm; gb2.max_n = mn; gb2.max_k = mk;
-  gemm_fp64_run(c, gb2, tile_cfg);
+  const KWindows W = plan_kwindows(c, A, ta, B, tb, R, bouts.as<GemmOut>());
+  c->last_kwindows = W.nwin;
+  for (int w = 0; w < W.nwin; ++w) {
+    GemmBatch gw = gb2;
+    gw.outs = W.outs + W.off[(size_t)w];
+    gw.nouts = W.off[(size_t)w + 1] - W.off[(size_t)w];
+    if (gw.nouts > 0) gemm_fp64_run(c, gw, tile_cfg);
+  }
   return R.ntasks;
+}
+
+// ------------------------------------------------------------- k-windows --
+__global__ void k_kb_bytes(const int64_t* __restrict__ off, const int32_t* __restrict__ rsz,
+                           const int32_t* __restrict__ csz, int32_t nrb, int64_t nb, int kb_is_row,
+                           unsigned long long* __restrict__ bytes) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb || off[b] < 0) return;
+  const int r = (int)(b % nrb), cc = (int)(b / nrb);
+  atomicAdd(bytes + (kb_is_row ? r : cc), (unsigned long long)rsz[r] * (unsigned long long)csz[cc] * 8ull);
+}
+
+__global__ void k_out_bytes(const GemmOut* __restrict__ outs, int64_t nouts, unsigned long long* __restrict__ total) {
+  const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= nouts) return;
+  atomicAdd(total, (unsigned long long)outs[o].m * (unsigned long long)outs[o].n * 8ull);
+}
+
+// flags[w * nouts + o] = 1 when output o has a product in window w
+__global__ void k_win_flags(const GemmOut* __restrict__ outs, int64_t nouts, const int32_t* __restrict__ kb,
+                            const int32_t* __restrict__ win_of_kb, int64_t* __restrict__ flags) {
+  const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= nouts) return;
+  int last = -1;
+  for (int32_t p = outs[o].p0; p < outs[o].p1; ++p) {
+    const int w = win_of_kb[kb[p]];
+    if (w != last) flags[(int64_t)w * nouts + o] = 1;
+    last = w;
+  }
+}
+
+__global__ void k_win_emit(const GemmOut* __restrict__ outs, int64_t nouts, const int32_t* __restrict__ kb,
+                           const int32_t* __restrict__ win_of_kb, const int64_t* __restrict__ pos,
+                           GemmOut* __restrict__ wouts) {
+  const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= nouts) return;
+  const GemmOut g = outs[o];
+  int32_t p = g.p0;
+  bool first = true;
+  while (p < g.p1) {
+    const int w = win_of_kb[kb[p]];
+    int32_t q = p + 1;
+    while (q < g.p1 && win_of_kb[kb[q]] == w) ++q;
+    GemmOut x = g;
+    x.p0 = p;
+    x.p1 = q;
+    x.acc = first ? g.acc : 1;
+    wouts[pos[(int64_t)w * nouts + o]] = x;
+    first = false;
+    p = q;
+  }
+}
+
+__global__ void k_gather_stride(const int64_t* __restrict__ src, int64_t stride, int64_t n, int64_t* __restrict__ dst) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i * stride];
+}
+
+static int64_t env_mb(const char* name, int64_t dflt) {
+  const char* v = std::getenv(name);
+  return v ? std::atoll(v) : dflt;
+}
+
+KWindows plan_kwindows(sa_context* c, sa_bsm* A, bool ta, sa_bsm* B, bool tb, const ScreenResult& R,
+                       const GemmOut* outs) {
+  KWindows W;
+  W.outs = outs;
+  W.off = {0, R.nouts};
+  // operand budget per window and the cache size the C blocks must fit in
+  const int64_t budget = env_mb("SA_KWINDOW_MB", 160) << 20;
+  const int64_t cache = (int64_t)256 << 20;
+  const int32_t nkb = R.nkb;
+  if (budget <= 0 || nkb < 2 || R.nouts == 0) return W;
+  DevBuf& bb = c->buf("kw_bytes");
+  bb.reserve(sizeof(unsigned long long) * (size_t)(nkb + 1));
+  unsigned long long* kbytes = bb.as<unsigned long long>();
+  fill_i64(c, reinterpret_cast<int64_t*>(kbytes), nkb + 1, 0);
+  hipLaunchKernelGGL(k_kb_bytes, dim3((unsigned)ceil_div(std::max<int64_t>(A->nb(), 1), 256)), dim3(256), 0,
+                     c->stream, (const int64_t*)A->d_off, (const int32_t*)A->d_rsz, (const int32_t*)A->d_csz,
+                     A->nrb, A->nb(), ta ? 1 : 0, kbytes);
+  hipLaunchKernelGGL(k_kb_bytes, dim3((unsigned)ceil_div(std::max<int64_t>(B->nb(), 1), 256)), dim3(256), 0,
+                     c->stream, (const int64_t*)B->d_off, (const int32_t*)B->d_rsz, (const int32_t*)B->d_csz,
+                     B->nrb, B->nb(), tb ? 0 : 1, kbytes);
+  hipLaunchKernelGGL(k_out_bytes, dim3((unsigned)ceil_div(R.nouts, 256)), dim3(256), 0, c->stream, outs, R.nouts,
+                     kbytes + nkb);
+  std::vector<unsigned long long> hb((size_t)nkb + 1);
+  SA_CHECK_HIP(hipMemcpyAsync(hb.data(), kbytes, sizeof(unsigned long long) * (size_t)(nkb + 1),
+                              hipMemcpyDeviceToHost, c->stream));
+  SA_CHECK_HIP(hipStreamSynchronize(c->stream));
+  unsigned long long opnd = 0;
+  for (int32_t k = 0; k < nkb; ++k) opnd += hb[(size_t)k];
+  if ((int64_t)hb[(size_t)nkb] > cache || (int64_t)opnd <= 2 * budget) return W;
+  std::vector<int32_t> win((size_t)nkb);
+  int nwin = 0;
+  unsigned long long acc = 0;
+  for (int32_t k = 0; k < nkb; ++k) {
+    if (acc > 0 && acc + hb[(size_t)k] > (unsigned long long)budget) {
+      ++nwin;
+      acc = 0;
+    }
+    acc += hb[(size_t)k];
+    win[(size_t)k] = nwin;
+  }
+  ++nwin;
+  if (nwin < 2) return W;
+  DevBuf& bw = c->buf("kw_win");
+  DevBuf& bf = c->buf("kw_flags");
+  DevBuf& bo = c->buf("kw_outs");
+  DevBuf& bofs = c->buf("kw_off");
+  const int64_t nf = (int64_t)nwin * R.nouts;
+  bw.reserve(sizeof(int32_t) * (size_t)nkb);
+  bf.reserve(sizeof(int64_t) * (size_t)(nf + 1));
+  bofs.reserve(sizeof(int64_t) * (size_t)(nwin + 1));
+  SA_CHECK_HIP(hipMemcpyAsync(bw.p, win.data(), sizeof(int32_t) * (size_t)nkb, hipMemcpyHostToDevice, c->stream));
+  fill_i64(c, bf.as<int64_t>(), nf + 1, 0);
+  hipLaunchKernelGGL(k_win_flags, dim3((unsigned)ceil_div(R.nouts, 256)), dim3(256), 0, c->stream, outs, R.nouts,
+                     R.kb, (const int32_t*)bw.p, bf.as<int64_t>());
+  const int64_t total = scan_exclusive_i64(c, bf.as<int64_t>(), nf + 1);
+  bo.reserve(sizeof(GemmOut) * (size_t)(total + 1));
+  hipLaunchKernelGGL(k_win_emit, dim3((unsigned)ceil_div(R.nouts, 256)), dim3(256), 0, c->stream, outs, R.nouts,
+                     R.kb, (const int32_t*)bw.p, (const int64_t*)bf.p, bo.as<GemmOut>());
+  hipLaunchKernelGGL(k_gather_stride, dim3((unsigned)ceil_div(nwin + 1, 256)), dim3(256), 0, c->stream,
+                     (const int64_t*)bf.p, R.nouts, (int64_t)(nwin + 1), bofs.as<int64_t>());
+  W.off.assign((size_t)nwin + 1, 0);
+  SA_CHECK_HIP(hipMemcpyAsync(W.off.data(), bofs.p, sizeof(int64_t) * (size_t)(nwin + 1), hipMemcpyDeviceToHost,
+                              c->stream));
+  SA_CHECK_HIP(hipStreamSynchronize(c->stream));
+  SA_CHECK_HIP(hipGetLastError());
+  W.nwin = nwin;
+  W.outs = bo.as<GemmOut>();
+  return W;
 }
 
 // blocksparsematrix.hpp:299-305

@@ -208,7 +208,7 @@ __global__ void k_oz_records(OzTables t, GemmOut* __restrict__ outs, OzProd* __r
   o.p1 = (int32_t)(seg + cnt);
   o.acc = fresh ? 0 : 1;
   o.pad = 0;
-  outs[t.out_scan[c]] = o;
+  outs[t.out_scan[out_slot(ib, jb, t.nib)]] = o;
   for (int64_t p = seg; p < seg + cnt; ++p) {
     const int kb = t.kb[p];
     const int64_t ai = t.ta ? (int64_t)kb + (int64_t)ib * t.a_nrb : (int64_t)ib + (int64_t)kb * t.a_nrb;
@@ -233,7 +233,9 @@ __global__ void k_oz_records(OzTables t, GemmOut* __restrict__ outs, OzProd* __r
 // 1 KB, double-buffered in LDS.  The stream of stages runs over the products
 // of the output; the global loads of stage s+1 are in flight while the MFMAs of
 // stage s run, with one barrier per stage.  Fragments of 16-row blocks beyond
-// the output block are skipped (wave-uniform).
+// the output block hold zeros (their loads are skipped) and run through the
+// MFMAs like the others: a branch per fragment would put every MFMA in its own
+// basic block and serialise it behind the LDS reads.
 template <int S, bool BF, int FM, int FN>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
     k_gemm_ozaki(const GemmOut* __restrict__ outs, const OzProd* __restrict__ prods,
@@ -246,12 +248,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
   __shared__ uint4 As[2][NCA][64];
   __shared__ uint4 Bs[2][NCB][64];
 
-  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
   const int wm = wid & 1, wn = wid >> 1;
   const int li = lane & 15, lg = lane >> 4;
   const int64_t it = xcd_remap(blockIdx.x, nitems);
-  const GemmItem item = items ? items[it] : GemmItem{(int32_t)it, 0, 0};
-  const GemmOut o = outs[item.out];
+  const GemmItem item = items ? gload1(items + it) : GemmItem{(int32_t)it, 0, 0};
+  const GemmOut o = gload1(outs + item.out);
   const int m = o.m, n = o.n;
   const int nrb = (int)ceil_div(m, 16), ncb = (int)ceil_div(n, 16);
   const int rb0 = item.tm * RA, cb0 = item.tn * RB;
@@ -282,13 +285,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
     for (int r = 0; r < LA; ++r) {
       const int c = wid + 4 * r;
       const int p = c / RA, vb = rb0 + c % RA;
-      ra[r] = (c < NCA && vb < nrb) ? a4[p * psa + ((int64_t)vb * pr.nks + ks) * 64 + lane] : make_uint4(0, 0, 0, 0);
+      ra[r] = (c < NCA && vb < nrb) ? gload1(a4 + p * psa + ((int64_t)vb * pr.nks + ks) * 64 + lane)
+                                    : make_uint4(0, 0, 0, 0);
     }
 #pragma unroll
     for (int r = 0; r < LB; ++r) {
       const int c = wid + 4 * r;
       const int p = c / RB, vb = cb0 + c % RB;
-      rbf[r] = (c < NCB && vb < ncb) ? b4[p * psb + ((int64_t)vb * pr.nks + ks) * 64 + lane] : make_uint4(0, 0, 0, 0);
+      rbf[r] = (c < NCB && vb < ncb) ? gload1(b4 + p * psb + ((int64_t)vb * pr.nks + ks) * 64 + lane)
+                                     : make_uint4(0, 0, 0, 0);
     }
   };
   auto lstore = [&](int buf) {
@@ -307,9 +312,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) er[i][v] = act_i[i] ? pr.ea[(rb0 + wm * FM + i) * 16 + 4 * lg + v] : 0;
+      for (int v = 0; v < 4; ++v) er[i][v] = act_i[i] ? gload1(pr.ea + (rb0 + wm * FM + i) * 16 + 4 * lg + v) : 0;
 #pragma unroll
-    for (int j = 0; j < FN; ++j) ec[j] = act_j[j] ? pr.eb[(cb0 + wn * FN + j) * 16 + li] : 0;
+    for (int j = 0; j < FN; ++j) ec[j] = act_j[j] ? gload1(pr.eb + (cb0 + wn * FN + j) * 16 + li) : 0;
   };
   auto compute = [&](int buf) {
     uint4 bq[S][FN];
@@ -328,7 +333,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
         for (int i = 0; i < FM; ++i)
 #pragma unroll
           for (int j = 0; j < FN; ++j) {
-            if (!(act_i[i] && act_j[j])) continue;
             if (BF)
               s32[pa + pb][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
                   __builtin_bit_cast(bf16x8, aq[i]), __builtin_bit_cast(bf16x8, bq[pb][j]), s32[pa + pb][i][j], 0, 0, 0);
@@ -339,28 +343,29 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
     }
   };
   // acc += sum_{w in [W0, W1)} s_w 2^(e_r + f_c + 2 TS - beta (w + 2)); s_w = 0
-  auto fold = [&](auto w0c, auto w1c) {
-    constexpr int W0 = decltype(w0c)::value, W1 = decltype(w1c)::value;
+  // acc += s_w 2^(e_r + f_c + 2 TS - beta (w + 2)) and s_w = 0 for w in [lo, hi)
+  // (lo, hi wave-uniform; one call site, so no weight is ever indexed at run time)
+  auto fold = [&](int lo, int hi) {
 #pragma unroll
-    for (int i = 0; i < FM; ++i)
+    for (int w = 0; w < S; ++w) {
+      if (w < lo || w >= hi) continue;
 #pragma unroll
-      for (int j = 0; j < FN; ++j) {
-        if (!(act_i[i] && act_j[j])) continue;
+      for (int i = 0; i < FM; ++i)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          double t = 0.0;
+        for (int j = 0; j < FN; ++j) {
 #pragma unroll
-          for (int w = W0; w < W1; ++w) t = fma((double)s32[w][i][j][v], pow2i(-BETA * w), t);
-          acc[i][j][v] += ldexp(t, er[i][v] + ec[j] + 2 * TS - 2 * BETA);
+          for (int v = 0; v < 4; ++v)
+            acc[i][j][v] += ldexp((double)s32[w][i][j][v], er[i][v] + ec[j] + 2 * TS - BETA * (w + 2));
+          s32[w][i][j] = flt4{0.f, 0.f, 0.f, 0.f};
         }
-#pragma unroll
-        for (int w = W0; w < W1; ++w) s32[w][i][j] = flt4{0.f, 0.f, 0.f, 0.f};
-      }
+    }
   };
 
   int p = o.p0;
   if (p < o.p1) {
-    OzProd pr = prods[p];
+    OzProd pr = gload1(prods + p);
+    // record of the following product, loaded one product ahead
+    OzProd pf = p + 1 < o.p1 ? gload1(prods + p + 1) : pr;
     int ks = 0;
     load_exps(pr);
     gload(pr, 0);
@@ -376,16 +381,23 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
       const bool has_next = np < o.p1;
       OzProd pn = pr;
       if (has_next) {
-        if (np != p) pn = prods[np];
+        if (np != p) {
+          pn = pf;
+          if (np + 1 < o.p1) pf = gload1(prods + np + 1);
+        }
         gload(pn, nks);
       }
       compute(buf);
       const int done = ks + 1;
+      int lo = S, hi = 0;
       if (np != p || (done & 7) == 0) {
-        fold(std::integral_constant<int, 0>{}, std::integral_constant<int, S>{});
+        lo = 0;
+        hi = S;
       } else if (FREQ_HI > FREQ_LO && (done & 1) == 0) {
-        fold(std::integral_constant<int, FREQ_LO>{}, std::integral_constant<int, FREQ_HI>{});
+        lo = FREQ_LO;
+        hi = FREQ_HI;
       }
+      if (lo < hi) fold(lo, hi);
       if (!has_next) break;
       if (np != p) load_exps(pn);
       lstore(buf ^ 1);
@@ -417,22 +429,248 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
     }
 }
 
+// Deep-pipelined variant for large blocks: WM x WN waves (8: two per SIMD) of
+// (16 FM) x (16 FN) tiles, and the loads of the next TWO stages in flight while
+// a stage computes: stage s computes from LDS buffer s%2, register set (s+1)%2
+// holds stage s+1 (in flight since the previous stage) and the loads of stage
+// s+2 go to register set s%2.  The larger workgroup tile also cuts the operand
+// bytes per flop (128x64: 48 KB per 32-k stage for 2x the work of 64x64).
+struct OzCursor {
+  const uint4* a;
+  const uint4* b;
+  int32_t p, ks, nks;
+};
+
+template <int S, bool BF, int WM, int WN, int FM, int FN>
+__global__ void __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_eu(2, 2)))
+    k_gemm_ozaki_deep(const GemmOut* __restrict__ outs, const OzProd* __restrict__ prods,
+                      const GemmItem* __restrict__ items, int64_t nitems, double alpha) {
+  constexpr int BETA = oz_beta(BF), TS = oz_tshift(BF);
+  constexpr int NW = WM * WN;
+  constexpr int RA = WM * FM, RB = WN * FN;
+  constexpr int NCA = S * RA, NCB = S * RB;
+  constexpr int LA = (NCA + NW - 1) / NW, LB = (NCB + NW - 1) / NW;
+  // FP16: weight 1 is exact for 2 k-steps of 32, BF16 weights 0..2 for 8
+  constexpr int FOLD_EVERY = BF ? 8 : 2;
+  __shared__ uint4 As[2][NCA][64];
+  __shared__ uint4 Bs[2][NCB][64];
+
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wm = wid % WM, wn = wid / WM;
+  const int li = lane & 15, lg = lane >> 4;
+  const int64_t it = xcd_remap(blockIdx.x, nitems);
+  const GemmItem item = items ? gload1(items + it) : GemmItem{(int32_t)it, 0, 0};
+  const GemmOut o = gload1(outs + item.out);
+  const int m = o.m, n = o.n;
+  const int nrb = (int)ceil_div(m, 16), ncb = (int)ceil_div(n, 16);
+  const int rb0 = item.tm * RA, cb0 = item.tn * RB;
+  bool act_i[FM], act_j[FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i) act_i[i] = rb0 + wm * FM + i < nrb;
+#pragma unroll
+  for (int j = 0; j < FN; ++j) act_j[j] = cb0 + wn * FN + j < ncb;
+
+  dbl4 acc[FM][FN];
+  flt4 s32[S][FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      acc[i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int w = 0; w < S; ++w) s32[w][i][j] = flt4{0.f, 0.f, 0.f, 0.f};
+    }
+  int er[FM][4], ec[FN];
+  uint4 ra[2][LA], rbf[2][LB];
+  using I0 = std::integral_constant<int, 0>;
+  using I1 = std::integral_constant<int, 1>;
+
+  auto gload = [&](auto par, const OzCursor& c) {
+    constexpr int P = decltype(par)::value;
+    const int64_t psa = (int64_t)nrb * c.nks * 64, psb = (int64_t)ncb * c.nks * 64;
+#pragma unroll
+    for (int r = 0; r < LA; ++r) {
+      const int ci = wid + NW * r;
+      const int p = ci / RA, vb = rb0 + ci % RA;
+      ra[P][r] = (ci < NCA && vb < nrb) ? gload1(c.a + p * psa + ((int64_t)vb * c.nks + c.ks) * 64 + lane)
+                                        : make_uint4(0, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < LB; ++r) {
+      const int ci = wid + NW * r;
+      const int p = ci / RB, vb = cb0 + ci % RB;
+      rbf[P][r] = (ci < NCB && vb < ncb) ? gload1(c.b + p * psb + ((int64_t)vb * c.nks + c.ks) * 64 + lane)
+                                         : make_uint4(0, 0, 0, 0);
+    }
+  };
+  auto lstore = [&](auto par) {
+    constexpr int P = decltype(par)::value;
+#pragma unroll
+    for (int r = 0; r < LA; ++r) {
+      const int ci = wid + NW * r;
+      if (ci < NCA) As[P][ci][lane] = ra[P][r];
+    }
+#pragma unroll
+    for (int r = 0; r < LB; ++r) {
+      const int ci = wid + NW * r;
+      if (ci < NCB) Bs[P][ci][lane] = rbf[P][r];
+    }
+  };
+  // exponents of product p (record re-read: cursors carry only the slice pointers)
+  auto load_exps = [&](int p) {
+    const int32_t* ea = gload1(&prods[p].ea);
+    const int32_t* eb = gload1(&prods[p].eb);
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) er[i][v] = act_i[i] ? gload1(ea + (rb0 + wm * FM + i) * 16 + 4 * lg + v) : 0;
+#pragma unroll
+    for (int j = 0; j < FN; ++j) ec[j] = act_j[j] ? gload1(eb + (cb0 + wn * FN + j) * 16 + li) : 0;
+  };
+  auto compute = [&](int buf) {
+    uint4 bq[S][FN];
+#pragma unroll
+    for (int q = 0; q < S; ++q)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) bq[q][j] = Bs[buf][q * RB + wn * FN + j][lane];
+#pragma unroll
+    for (int pa = 0; pa < S; ++pa) {
+      uint4 aq[FM];
+#pragma unroll
+      for (int i = 0; i < FM; ++i) aq[i] = As[buf][pa * RA + wm * FM + i][lane];
+#pragma unroll
+      for (int pb = 0; pb < S - pa; ++pb)
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j) {
+            if (BF)
+              s32[pa + pb][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                  __builtin_bit_cast(bf16x8, aq[i]), __builtin_bit_cast(bf16x8, bq[pb][j]), s32[pa + pb][i][j], 0, 0, 0);
+            else
+              s32[pa + pb][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+                  __builtin_bit_cast(half8, aq[i]), __builtin_bit_cast(half8, bq[pb][j]), s32[pa + pb][i][j], 0, 0, 0);
+          }
+    }
+  };
+  // acc += sum_w s_w 2^(e_r + f_c + 2 TS - beta (w + 2)); s = 0.  All weights
+  // at every fold point (a compile-time range: the two parity copies of the
+  // stage body would otherwise index the weights at run time).
+  auto fold = [&]() {
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          double t = 0.0;
+#pragma unroll
+          for (int w = 0; w < S; ++w) t = fma((double)s32[w][i][j][v], pow2i(-BETA * w), t);
+          acc[i][j][v] += ldexp(t, er[i][v] + ec[j] + 2 * TS - 2 * BETA);
+        }
+#pragma unroll
+        for (int w = 0; w < S; ++w) s32[w][i][j] = flt4{0.f, 0.f, 0.f, 0.f};
+      }
+  };
+
+  const int p0 = o.p0, p1 = o.p1;
+  if (p0 < p1) {
+    auto mk = [&](int p) {
+      return OzCursor{reinterpret_cast<const uint4*>(gload1(&prods[p].a)),
+                      reinterpret_cast<const uint4*>(gload1(&prods[p].b)), p, 0, gload1(&prods[p].nks)};
+    };
+    OzCursor pf = mk(p0 + 1 < p1 ? p0 + 1 : p0);  // product after the furthest cursor's
+    // next stage after c (the furthest cursor); false at the end of the stream
+    auto adv = [&](const OzCursor& c, OzCursor& nx) -> bool {
+      if (c.ks + 1 < c.nks) {
+        nx = c;
+        nx.ks = c.ks + 1;
+        return true;
+      }
+      if (c.p + 1 >= p1) return false;
+      nx = pf;
+      if (c.p + 2 < p1) pf = mk(c.p + 2);
+      return true;
+    };
+    OzCursor cur = mk(p0), c1 = cur;
+    bool v1 = adv(cur, c1);
+    load_exps(p0);
+    gload(I0{}, cur);
+    lstore(I0{});
+    if (v1) gload(I1{}, c1);
+    __syncthreads();
+    // stage with parity P: LDS buffer P holds cur, register set P^1 holds c1
+    auto step = [&](auto par) -> bool {
+      constexpr int P = decltype(par)::value;
+      OzCursor c2 = c1;
+      bool v2 = false;
+      if (v1) {
+        v2 = adv(c1, c2);
+        if (v2) gload(par, c2);
+      }
+      compute(P);
+      const int done = cur.ks + 1;
+      if (done == cur.nks || (done & (FOLD_EVERY - 1)) == 0) fold();
+      if (!v1) return true;
+      lstore(std::integral_constant<int, P ^ 1>{});
+      __syncthreads();
+      if (c1.p != cur.p) load_exps(c1.p);
+      cur = c1;
+      c1 = c2;
+      v1 = v2;
+      return false;
+    };
+    while (true) {
+      if (step(I0{})) break;
+      if (step(I1{})) break;
+    }
+  }
+
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      if (!(act_i[i] && act_j[j])) continue;
+      const int col = (cb0 + wn * FN + j) * 16 + li;
+      if (col >= n) continue;
+      double* cp = o.c + (int64_t)col * m;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = (rb0 + wm * FM + i) * 16 + 4 * lg + v;
+        if (row < m) {
+          double x = alpha * acc[i][j][v];
+          if (o.acc) x = x + cp[row];
+          cp[row] = x;
+        }
+      }
+    }
+}
+
 using OzFn = void (*)(const GemmOut*, const OzProd*, const GemmItem*, int64_t, double);
 
 struct OzCfg {
   int S;
   bool bf;
   int fm, fn;
+  bool deep;
+  int threads;
   OzFn fn_;
 };
 
-#define SA_OZ(S, BF, FM, FN) OzCfg{S, BF, FM, FN, k_gemm_ozaki<S, BF, FM, FN>}
+#define SA_OZ(S, BF, FM, FN) OzCfg{S, BF, FM, FN, false, 256, k_gemm_ozaki<S, BF, FM, FN>}
+// deep: (WM x WN) = (RM x RN) waves of 32 x 32, workgroup tile (32 RM) x (32 RN)
+#define SA_OZD(S, BF, RM, RN) OzCfg{S, BF, RM, RN, true, 64 * RM * RN, k_gemm_ozaki_deep<S, BF, RM, RN, 2, 2>}
 
 static const OzCfg kOzCfgs[] = {
-    SA_OZ(3, false, 2, 2), SA_OZ(4, false, 2, 2), SA_OZ(5, false, 2, 2),
-    SA_OZ(3, false, 1, 1), SA_OZ(4, false, 1, 1), SA_OZ(5, false, 1, 1),
-    SA_OZ(4, true, 2, 2),  SA_OZ(5, true, 2, 2),  SA_OZ(6, true, 2, 2),
-    SA_OZ(4, true, 1, 1),  SA_OZ(5, true, 1, 1),  SA_OZ(6, true, 1, 1),
+    SA_OZ(3, false, 2, 2),  SA_OZ(4, false, 2, 2),  SA_OZ(5, false, 2, 2),   // 0-2: 64x64
+    SA_OZ(3, false, 1, 1),  SA_OZ(4, false, 1, 1),  SA_OZ(5, false, 1, 1),   // 3-5: 32x32
+    SA_OZ(4, true, 2, 2),   SA_OZ(5, true, 2, 2),   SA_OZ(6, true, 2, 2),    // 6-8
+    SA_OZ(4, true, 1, 1),   SA_OZ(5, true, 1, 1),   SA_OZ(6, true, 1, 1),    // 9-11
+    SA_OZD(3, false, 4, 2), SA_OZD(4, false, 4, 2), SA_OZD(5, false, 4, 2),  // 12-14: 128x64, deep
+    SA_OZD(3, false, 2, 4), SA_OZD(4, false, 2, 4), SA_OZD(5, false, 2, 4),  // 15-17: 64x128, deep
+    SA_OZD(4, true, 4, 2),  SA_OZD(5, true, 4, 2),                           // 18-19
+    SA_OZD(4, true, 2, 4),  SA_OZD(5, true, 2, 4),                           // 20-21
 };
 
 __global__ void k_count_tiles_oz(const GemmOut* __restrict__ outs, int64_t nouts, int TM, int TN,
@@ -497,13 +735,35 @@ static Sliced slice_operand(sa_context* c, sa_bsm* X, bool role_b, bool trans, i
   return Sliced{bsl.as<uint16_t>(), bso.as<int64_t>(), bex.as<int32_t>(), beo.as<int64_t>()};
 }
 
-static int pick_cfg(int S, bool bf, int max_m, int max_n, int tile) {
-  const int want_fm = (std::max(max_m, max_n) > 32) ? 2 : 1;
-  for (int i = 0; i < (int)(sizeof(kOzCfgs) / sizeof(kOzCfgs[0])); ++i) {
+static constexpr int kNumOzCfgs = (int)(sizeof(kOzCfgs) / sizeof(kOzCfgs[0]));
+
+// Padded-tile work of one tile size over the C block grid (separable in rows
+// and columns), relative to the true block area.
+static double padded_ratio(const std::vector<int32_t>& rsz, const std::vector<int32_t>& csz, int TM, int TN) {
+  double r = 0, rp = 0, c = 0, cp = 0;
+  for (int32_t v : rsz) { r += v; rp += (double)ceil_div(v, TM) * TM; }
+  for (int32_t v : csz) { c += v; cp += (double)ceil_div(v, TN) * TN; }
+  return (r > 0 && c > 0) ? (rp / r) * (cp / c) : 1.0;
+}
+
+// Kernel choice: a forced table id that runs this slice count/type, else the
+// lowest padded work weighted by the relative cost per useful flop of each
+// kernel shape (deep 128-wide tiles < 64x64 < 32x32).
+static int pick_cfg(int S, bool bf, const sa_bsm* C, int tile) {
+  if (tile >= 0 && tile < kNumOzCfgs && kOzCfgs[tile].S == S && kOzCfgs[tile].bf == bf) return tile;
+  int best = -1;
+  double best_cost = 1e300;
+  for (int i = 0; i < kNumOzCfgs; ++i) {
     const OzCfg& k = kOzCfgs[i];
-    if (k.S == S && k.bf == bf && (tile < 0 ? k.fm == want_fm : i == tile)) return i;
+    if (k.S != S || k.bf != bf) continue;
+    const double w = k.deep ? 0.7 : (k.fm == 2 ? 1.0 : 1.6);
+    const double cost = w * padded_ratio(C->rsz, C->csz, 32 * k.fm, 32 * k.fn);
+    if (cost < best_cost) {
+      best_cost = cost;
+      best = i;
+    }
   }
-  return -1;
+  return best;
 }
 
 int64_t ozaki_multiply(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, double thresh, double alpha,
@@ -511,7 +771,7 @@ int64_t ozaki_multiply(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, double
   sa_context* c = C->ctx;
   const bool bf = o.slice_type == SA_SLICE_BF16;
   const int S = o.slices;
-  const int cfg = pick_cfg(S, bf, 64, 64, -1);
+  const int cfg = pick_cfg(S, bf, C, -1);
   SA_REQUIRE(cfg >= 0, "ozaki: %d %s slices not supported (FP16: 3..5, BF16: 4..6)", S, bf ? "BF16" : "FP16");
   const ScreenResult R = screen_and_allocate(C, A, ta, B, tb, thresh, nullptr, nullptr, 0, false);
   if (R.ntasks == 0) return 0;
@@ -542,28 +802,32 @@ int64_t ozaki_multiply(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, double
   int32_t mm = 0, mn = 0;
   for (int32_t v : C->rsz) mm = std::max(mm, v);
   for (int32_t v : C->csz) mn = std::max(mn, v);
-  int cf = o.tile >= 0 ? pick_cfg(S, bf, mm, mn, o.tile) : -1;  // a forced id that runs other slices is ignored
-  if (cf < 0) cf = pick_cfg(S, bf, mm, mn, -1);
+  const int cf = pick_cfg(S, bf, C, o.tile);  // a forced id that runs other slices is ignored
   const OzCfg& k = kOzCfgs[cf];
   const int TM = 32 * k.fm, TN = 32 * k.fn;
-  const GemmItem* items = nullptr;
-  int64_t nitems = R.nouts;
-  if (mm > TM || mn > TN) {
-    c->ws_gemm_items.reserve(sizeof(int64_t) * (size_t)(R.nouts + 1));
-    int64_t* cnt = c->ws_gemm_items.as<int64_t>();
-    hipLaunchKernelGGL(k_count_tiles_oz, dim3((unsigned)ceil_div(R.nouts, 256)), dim3(256), 0, c->stream,
-                       bouts.as<GemmOut>(), R.nouts, TM, TN, cnt);
-    nitems = scan_exclusive_i64(c, cnt, R.nouts);
-    c->ws_gemm_items2.reserve(sizeof(GemmItem) * (size_t)(nitems + 1));
-    hipLaunchKernelGGL(k_emit_items_oz, dim3((unsigned)ceil_div(R.nouts, 256)), dim3(256), 0, c->stream,
-                       bouts.as<GemmOut>(), R.nouts, TM, TN, (const int64_t*)cnt, c->ws_gemm_items2.as<GemmItem>());
-    items = c->ws_gemm_items2.as<GemmItem>();
-  }
-  SA_REQUIRE(nitems < ((int64_t)1 << 31), "ozaki: too many work items");
-  {
+  const KWindows W = plan_kwindows(c, A, ta, B, tb, R, bouts.as<GemmOut>());
+  c->last_kwindows = W.nwin;
+  for (int w = 0; w < W.nwin; ++w) {
+    const GemmOut* wo = W.outs + W.off[(size_t)w];
+    const int64_t nw = W.off[(size_t)w + 1] - W.off[(size_t)w];
+    if (nw == 0) continue;
+    const GemmItem* items = nullptr;
+    int64_t nitems = nw;
+    if (mm > TM || mn > TN) {
+      c->ws_gemm_items.reserve(sizeof(int64_t) * (size_t)(nw + 1));
+      int64_t* cnt = c->ws_gemm_items.as<int64_t>();
+      hipLaunchKernelGGL(k_count_tiles_oz, dim3((unsigned)ceil_div(nw, 256)), dim3(256), 0, c->stream, wo, nw, TM,
+                         TN, cnt);
+      nitems = scan_exclusive_i64(c, cnt, nw);
+      c->ws_gemm_items2.reserve(sizeof(GemmItem) * (size_t)(nitems + 1));
+      hipLaunchKernelGGL(k_emit_items_oz, dim3((unsigned)ceil_div(nw, 256)), dim3(256), 0, c->stream, wo, nw, TM,
+                         TN, (const int64_t*)cnt, c->ws_gemm_items2.as<GemmItem>());
+      items = c->ws_gemm_items2.as<GemmItem>();
+    }
+    SA_REQUIRE(nitems < ((int64_t)1 << 31), "ozaki: too many work items");
     ProfScope ps(c, "gemm_ozaki");
-    hipLaunchKernelGGL(k.fn_, dim3((unsigned)nitems), dim3(256), 0, c->stream,
-                       (const GemmOut*)bouts.as<GemmOut>(), (const OzProd*)bprods.as<OzProd>(), items, nitems, alpha);
+    hipLaunchKernelGGL(k.fn_, dim3((unsigned)nitems), dim3((unsigned)k.threads), 0, c->stream, wo,
+                       (const OzProd*)bprods.as<OzProd>(), items, nitems, alpha);
   }
   SA_CHECK_HIP(hipGetLastError());
   return R.ntasks;

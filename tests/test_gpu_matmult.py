@@ -145,3 +145,25 @@ def test_chained_products_use_stale_norms():
     oC.calc_norms(); dC.calc_norms()
     n, _ = check_mm(oD, dD, oC, dC, 0, oA, dA, 0, 1e-20, 1.0, 0.0)
     assert n == 64
+
+
+@pytest.mark.parametrize("beta", [0.0, 1.0])
+def test_kwindows_match_single_launch(monkeypatch, beta):
+    """A 1 MB operand budget splits the band product into many k-windows; the
+    task list is unchanged and C stays within the FP64 bar, with existing C
+    blocks (beta = 1) accumulated only in their first window."""
+    cfg = synth.config1(n=1024, bs=64, band=6)
+    a = synth.dense_from_scales(1, cfg.rsz, cfg.ksz, cfg.a_scale)
+    b = synth.dense_from_scales(2, cfg.ksz, cfg.csz, cfg.b_scale)
+    c0 = synth.dense_from_scales(3, cfg.rsz, cfg.csz, cfg.a_scale)
+    oA, dA = pair_uniform(a, 64, 64, cfg.thresh)
+    oB, dB = pair_uniform(b, 64, 64, cfg.thresh)
+    oC, dC = pair_uniform(c0, 64, 64, 0.0)
+    monkeypatch.setenv("SA_KWINDOW_MB", "1")
+    n, err = check_mm(oC, dC, oA, dA, 0, oB, dB, 0, cfg.thresh, 1.0, beta)
+    assert ctx().kwindows() > 4
+    monkeypatch.setenv("SA_KWINDOW_MB", "0")
+    oD, dD = empty_pair_uniform(1024, 1024, 64, 64)
+    check_mm(oD, dD, oA, dA, 0, oB, dB, 0, cfg.thresh, 1.0, 0.0)
+    assert ctx().kwindows() == 1
+

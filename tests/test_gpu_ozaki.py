@@ -83,6 +83,33 @@ def test_deep_k_fold_schedule(stype, slices):
     assert rel_fro(got, ref) < TOL
 
 
+DEEP = [(12, 3, sa.SLICE_FP16), (13, 4, sa.SLICE_FP16), (14, 5, sa.SLICE_FP16), (15, 3, sa.SLICE_FP16),
+        (16, 4, sa.SLICE_FP16), (17, 5, sa.SLICE_FP16), (18, 4, sa.SLICE_BF16), (19, 5, sa.SLICE_BF16),
+        (20, 4, sa.SLICE_BF16), (21, 5, sa.SLICE_BF16)]
+
+
+@pytest.mark.parametrize("tile,slices,stype", DEEP)
+@pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0)])
+def test_deep_pipelined_kernels(tile, slices, stype, ta, tb):
+    """128x64 / 64x128 deep-pipelined kernels on ragged 150-blocks (several
+    tiles, partial fragments, multi-product outputs and k-steps of 32)."""
+    bs = 150
+    rng = np.random.default_rng(500 + tile + ta)
+    ni, nk, nj = 2 * bs + 7, 3 * bs + 3, 2 * bs - 1
+    a_shape = (nk, ni) if ta else (ni, nk)
+    b_shape = (nj, nk) if tb else (nk, nj)
+    a = rng.standard_normal(a_shape)
+    b = rng.standard_normal(b_shape)
+    oA, dA = pair_uniform(a, bs, bs, 0.0)
+    oB, dB = pair_uniform(b, bs, bs, 0.0)
+    oC, dC = empty_pair_uniform(ni, nj, bs, bs)
+    ref, got = run(oA, dA, ta, oB, dB, tb, oC, dC, 0.0,
+                   sa.default_opts(mode=sa.MODE_OZAKI, slices=slices, slice_type=stype, tile=tile))
+    # bars: 2^-(beta S - TS) truncation; 3 FP16 slices (28 bits) and 4 BF16 slices (31 bits) are coarser
+    bound = {(3, sa.SLICE_FP16): 1e-7, (4, sa.SLICE_BF16): 1e-8}.get((slices, stype), TOL)
+    assert rel_fro(got, ref) < bound
+
+
 def test_integer_inputs_exact():
     """Inputs with at most 3 significant bits per row max live in the leading
     slice, so every dropped slice product is zero and the result is exact."""
@@ -114,3 +141,18 @@ def test_unsupported_slice_count_rejected():
     _, dC = empty_pair_uniform(32, 32, 16, 16)
     with pytest.raises(sa.SAError):
         sa.matmult(dC, dA, 0, dB, 0, 0.0, 1.0, 0.0, sa.default_opts(mode=sa.MODE_OZAKI, slices=9))
+
+
+def test_kwindows_split_precision(monkeypatch):
+    """k-windows under the split-precision kernel: same task list, same bar."""
+    cfg = synth.config1(n=1024, bs=64)
+    a = synth.dense_from_scales(1, cfg.rsz, cfg.ksz, cfg.a_scale)
+    b = synth.dense_from_scales(2, cfg.ksz, cfg.csz, cfg.b_scale)
+    oA, dA = pair_uniform(a, 64, 64, cfg.thresh)
+    oB, dB = pair_uniform(b, 64, 64, cfg.thresh)
+    oC, dC = empty_pair_uniform(1024, 1024, 64, 64)
+    monkeypatch.setenv("SA_KWINDOW_MB", "1")
+    ref, got = run(oA, dA, 0, oB, dB, 0, oC, dC, cfg.thresh, sa.default_opts(mode=sa.MODE_OZAKI, slices=4))
+    assert ctx().kwindows() > 4
+    assert rel_fro(got, ref) < TOL
+
