@@ -214,6 +214,66 @@ def bench_rik(args):
     print(json.dumps(rec))
 
 
+# ------------------------------------------------------ hybrid Strassen --
+def bench_hybrid(args):
+    """Large hybrid recursion: n=32768, 128x128 blocks, band |I-J|<=95 under
+    exp(-0.05|I-J|), cutoff 2048 (Strassen depth <= 4).  One step = one
+    hybrid multiply into a pre-filled C, planning (dry run, features, network,
+    plan) included; variants: plain screened multiply, the reference cost-model
+    tree, the learned plan."""
+    w = synth.config5()
+    n, bs, cutoff = w.n, w.uniform_bs, w.extra["cutoff"]
+    ctx = sa.Context(0)
+    A = sa.BSM.uniform(ctx, n, n, bs, bs, w.thresh).generate(1, w.a_scale)
+    B = sa.BSM.uniform(ctx, n, n, bs, bs, w.thresh).generate(2, w.b_scale)
+    Cm = sa.BSM.uniform(ctx, n, n, bs, bs, 0.0)
+    Cm.fill(0.0)
+    log(f"config5: A, B generated ({A.info()[5] / 1e9:.1f} GB each), C pre-filled ({Cm.info()[5] / 1e9:.1f} GB)")
+    model = os.path.exists(sa.DEFAULT_PATH_MODEL)
+    if model:
+        sa.mlp_load(ctx, sa.DEFAULT_PATH_MODEL)
+
+    variants = {
+        "screened": lambda: sa.matmult(Cm, A, 0, B, 0, w.thresh, 1.0, 1.0),
+        "heuristic": lambda: sa.matmult_strassen_sparse(Cm, A, B, w.thresh, cutoff,
+                                                        opts=sa.default_opts(scheduler=sa.SCHED_HEURISTIC)),
+        "learned": lambda: sa.matmult_strassen_sparse(Cm, A, B, w.thresh, cutoff,
+                                                      opts=sa.default_opts(scheduler=sa.SCHED_MLP)),
+    }
+    res, stats, outs = {}, {}, {}
+    ref = None
+    for name, fn in variants.items():
+        Cm.fill(0.0)
+        fn()
+        ctx.synchronize()
+        stats[name] = {**sa.strassen_last_stats(), **sa.strassen_last_plan()} if name != "screened" else \
+            {"tasks": int(len(ctx.tasklist()))}
+        outs[name] = Cm.frobenius_norm()
+        if name == "heuristic":
+            ref = Cm.to_dense() if not args.no_check else None
+        if name == "learned" and ref is not None:
+            stats[name]["rel_diff_vs_heuristic"] = rel(Cm.to_dense(), ref)
+        for _ in range(max(args.warmup - 1, 0)):
+            fn()
+        ms = timed(ctx, fn, args.steps)
+        res[name] = ms
+        log(f"config5 {name}: {ms:.1f} ms per step")
+    dense = 2.0 * n ** 3
+    best = "learned" if model else "heuristic"
+    rec = {
+        "metric": METRIC, "value": round(dense / (res[best] * 1e-3) / 1e12, 3), "unit": "dense-equivalent TFLOP/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(res[best], 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded counter-based generator, generated in HBM)",
+        "config": {"workload": w.name, "n": n, "bs": bs, "band": w.extra["band"], "decay": w.extra["decay"],
+                   "thresh": w.thresh, "cutoff": cutoff, "scheduler": best, "parallelism": "single GPU"},
+        "hybrid": {"ms_per_step": {k: round(v, 3) for k, v in res.items()},
+                   "learned_speedup_vs_heuristic": round(res["heuristic"] / res["learned"], 3),
+                   "model_loaded": model, "stats": stats},
+    }
+    print(json.dumps(rec))
+
+
 # -------------------------------------------------------- screened GEMM --
 def bench_gemm(args):
     if args.workload == "config3":
@@ -310,12 +370,15 @@ def main():
     ap.add_argument("--slices-k", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--n-occ", type=int, default=200, help="RI-K occupied orbitals (200 = the headline workload)")
+    ap.add_argument("--no-check", action="store_true", help="config5: skip the learned-vs-heuristic comparison")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1", "n_gpus": args.gpus}))
         sys.exit(2)
     if args.workload == "rik":
         bench_rik(args)
+    elif args.workload == "config5":
+        bench_hybrid(args)
     else:
         bench_gemm(args)
 

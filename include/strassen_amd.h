@@ -54,7 +54,11 @@ typedef struct {
   int32_t scheduler;   /* SA_SCHED_* (hybrid Strassen only) */
   int32_t reserved;
   int64_t cutoff;      /* min_size_for_strassen (main.cpp:9), elements */
-  const uint8_t* tree; /* SA_SCHED_TREE: bits in dry-run layout */
+  const uint8_t* tree; /* SA_SCHED_TREE: Strassen-root bits in dry-run layout */
+  const uint8_t* tree_dense; /* SA_SCHED_TREE (optional): dense-root bits, same layout; a
+                                dense root adds all present block products of its region,
+                                unscreened, as one library GEMM (no reference counterpart:
+                                the learned path selector's third path) */
 } sa_opts_t;
 
 void sa_opts_default(sa_opts_t* o);
@@ -213,9 +217,47 @@ int sa_matmult_strassen_sparse(sa_bsm_t C, sa_bsm_t A, int transA, sa_bsm_t B, i
                                double alpha, double beta, const sa_opts_t* opts);
 /* Statistics of the context-independent last hybrid call: executed Strassen
  * roots, Winograd nodes, dense leaf block products, peak scratch bytes of a
- * batch, batches, screened 8-path leaf products. */
+ * batch, batches, screened 8-path leaf products, dense roots. */
 int sa_strassen_last_stats(int64_t* nroots, int64_t* nnodes, int64_t* nleaf_products, int64_t* scratch_bytes,
-                           int64_t* batches, int64_t* screened);
+                           int64_t* batches, int64_t* screened, int64_t* dense_roots);
+
+/* ---------------------------------------------------- learned path selector --
+ * No counterpart in the reference, whose recursion is steered by the fixed
+ * cost model of matmult_strassen_dryrun (strassen.hpp:510-679).  A small MLP
+ * (kMlpFeatures node features -> 32 -> 32 (tanh) -> 2) predicts, per dry-run
+ * node, log2 of the cost ratio of the Strassen-Winograd and of the dense path
+ * against the screened-sparse path; a bottom-up pass over the tree picks the
+ * cheapest of descend / Strassen / dense per node (interior nodes only).  The
+ * heuristic is the fallback (no model loaded) and the judge: a learned choice
+ * is admissible only where the reference cost model prices it at no more than
+ * 1.5x the node's screened flops (Strassen: its node cost; dense: the region's
+ * dense flops), and a whole plan costing more than 4x the heuristic plan's
+ * model flops is overruled.  The screening threshold is never touched. */
+int sa_mlp_info(int32_t* n_features, int32_t* n_outputs, int64_t* n_params);
+/* Install a model: n_params parameters, feature means and scales (x' = (x-mu)/sd). */
+int sa_mlp_set(sa_ctx_t ctx, const double* params, int64_t n_params, const double* feat_mu, const double* feat_sd);
+int sa_mlp_get(sa_ctx_t ctx, double* params, double* feat_mu, double* feat_sd);
+/* Text model file: "path_mlp F 32 O P", P parameters, F means, F scales. */
+int sa_mlp_load(sa_ctx_t ctx, const char* path);
+/* Y (n x O, row-major) = model(X (n x F, row-major)) on the device. */
+int sa_mlp_forward(sa_ctx_t ctx, const double* X, int64_t n, double* Y);
+/* Gradient of L = 1/(2n) sum m (y - t)^2 w.r.t. the parameters (device
+ * backward kernels); M (n x O) masks missing targets, NULL = all present. */
+int sa_mlp_grad(sa_ctx_t ctx, const double* X, const double* T, const double* M, int64_t n, double* grad,
+                double* loss);
+/* epochs full-batch Adam steps on the device; loss_out = loss before the last step. */
+int sa_mlp_train(sa_ctx_t ctx, const double* X, const double* T, const double* M, int64_t n, int32_t epochs,
+                 double lr, double* loss_out);
+/* Features of every dry-run node of A*B (nnodes x F, dry-run order); C may be
+ * NULL (all C blocks present).  feat NULL: *nnodes only. */
+int sa_strassen_features(sa_bsm_t C, sa_bsm_t A, sa_bsm_t B, double thresh, double* feat, int64_t cap,
+                         int64_t* nnodes);
+/* The learned plan (Strassen-root and dense-root bits, dry-run layout) and the
+ * judge's verdict, without executing it. */
+int sa_strassen_plan(sa_bsm_t C, sa_bsm_t A, sa_bsm_t B, double thresh, uint8_t* sbits, uint8_t* dbits, int64_t cap,
+                     int32_t* overruled, double* model_cost, double* heuristic_cost);
+/* Schedule used by the last hybrid call (SA_SCHED_*) and the judge's figures. */
+int sa_strassen_last_plan(int32_t* scheduler, int32_t* overruled, double* model_cost, double* heuristic_cost);
 
 #ifdef __cplusplus
 }

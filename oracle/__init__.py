@@ -63,6 +63,7 @@ def lib():
             "or_strassen_dryrun": (C.c_int, [P, P, D, P, I64, P, P]),
             "or_matmult_strassen_sparse": (C.c_int, [P, P, P, D, I64, P, I64, P, P]),
             "or_matmult_strassen_tree": (C.c_int, [P, P, P, D, I64, P, P, I64, P, P]),
+            "or_matmult_strassen_plan": (C.c_int, [P, P, P, D, I64, P, P, P, I64, P, P, P]),
             "or_dgemm": (None, [C.c_int, C.c_int, I64, I64, I64, D, P, I64, P, I64, D, P, I64]),
             "or_occ_ri_k": (C.c_int, [I64, I64, I64, I64, P, P, P, I32, D, P]),
             "or_occ_ri_k_dense": (C.c_int, [I64, I64, I64, I64, P, P, P, P]),
@@ -249,18 +250,31 @@ def forced_tree(nib, levels):
     return [np.full(s, 1 if l in levels else 0, np.uint8) for l, s in enumerate(tree_level_sizes(nib))]
 
 
-def strassen_sparse(Cm, A, B, thresh, cutoff, tree=None):
+def _flat_tree(tree):
+    return np.ascontiguousarray(np.concatenate(tree) if len(tree) else np.zeros(1), dtype=np.uint8)
+
+
+def strassen_sparse(Cm, A, B, thresh, cutoff, tree=None, dense=None, counts=False):
+    """Hybrid multiply with the dry-run tree, or caller Strassen bits `tree` and
+    dense-root bits `dense` (per-level lists in dry-run layout).  Returns the
+    leaf task list and the number of Strassen roots (and dense roots when
+    counts=True)."""
     cap = int(A.nrb) ** 3
     leaf = np.zeros((max(cap, 1), 3), dtype=np.int32)
     nl = I64(0)
     nr = I64(0)
-    tb = None
-    if tree is not None:
-        tb = np.ascontiguousarray(np.concatenate(tree) if len(tree) else np.zeros(1), dtype=np.uint8)
-    rc = lib().or_matmult_strassen_tree(Cm.h, A.h, B.h, thresh, cutoff, None if tb is None else _ptr(tb),
-                                        _ptr(leaf), cap, C.byref(nl), C.byref(nr))
+    nd = I64(0)
+    tb = None if tree is None else _flat_tree(tree)
+    db = None if dense is None else _flat_tree(dense)
+    if tb is None and db is not None:
+        tb = np.zeros_like(db)
+    rc = lib().or_matmult_strassen_plan(Cm.h, A.h, B.h, thresh, cutoff, None if tb is None else _ptr(tb),
+                                        None if db is None else _ptr(db), _ptr(leaf), cap, C.byref(nl),
+                                        C.byref(nr), C.byref(nd))
     if rc != 0:
         raise ValueError(f"strassen failed {rc}")
+    if counts:
+        return leaf[: nl.value].copy(), nr.value, nd.value
     return leaf[: nl.value].copy(), nr.value
 
 

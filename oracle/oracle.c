@@ -676,6 +676,8 @@ typedef struct {
   or_bsm* C;
   const or_bsm *A, *B;
   const uint8_t* bits;
+  const uint8_t* dense;
+  int64_t ndense;
   const int64_t* lvl_off;
   double thr;
   sctx_t sc;
@@ -707,7 +709,26 @@ static void h_recurse(hctx_t* h, int64_t ib0, int64_t jb0, int64_t kb0, int64_t 
   }
   const int64_t nsi = ceil_div(nib, sb), nsj = ceil_div(njb, sb);
   const int64_t si = ib0 / sb, sj = jb0 / sb, sk = kb0 / sb;
-  const int go = h->bits[h->lvl_off[level] + IJK(si, sj, sk, nsi, nsj)];
+  const int64_t node = h->lvl_off[level] + IJK(si, sj, sk, nsi, nsj);
+  const int go = h->bits[node];
+  if (!go && h->dense && h->dense[node]) {
+    /* dense root: every present triple of the region, unscreened */
+    h->ndense++;
+    const int64_t ie = ib0 + sb < nib ? ib0 + sb : nib, je = jb0 + sb < njb ? jb0 + sb : njb;
+    const int64_t ke = kb0 + sb < nkb ? kb0 + sb : nkb;
+    for (int64_t jb = jb0; jb < je; ++jb)
+      for (int64_t ib = ib0; ib < ie; ++ib) {
+        double* cb = h->C->blk[BIDX(h->C, ib, jb)];
+        if (!cb) continue;
+        for (int64_t kb = kb0; kb < ke; ++kb) {
+          const double *ab = h->A->blk[BIDX(h->A, ib, kb)], *bb = h->B->blk[BIDX(h->B, kb, jb)];
+          if (!ab || !bb) continue;
+          const int64_t m = h->C->rsz[ib], n = h->C->csz[jb], k = h->A->csz[kb];
+          or_dgemm(0, 0, m, n, k, 1.0, ab, m, bb, k, 1.0, cb, m);
+        }
+      }
+    return;
+  }
   if (go) {
     const int64_t hh = sb / 2, bs2 = h->sc.bs * h->sc.bs;
     h->nroots++;
@@ -766,6 +787,14 @@ int or_matmult_strassen_sparse(or_bsm* C, const or_bsm* A, const or_bsm* B, doub
 int or_matmult_strassen_tree(or_bsm* C, const or_bsm* A, const or_bsm* B, double thresh,
                              int64_t min_size_for_strassen, const uint8_t* tree_bits,
                              int32_t* leaf_tasks, int64_t cap, int64_t* nleaf, int64_t* nroots) {
+  return or_matmult_strassen_plan(C, A, B, thresh, min_size_for_strassen, tree_bits, NULL, leaf_tasks, cap, nleaf,
+                                  nroots, NULL);
+}
+
+int or_matmult_strassen_plan(or_bsm* C, const or_bsm* A, const or_bsm* B, double thresh,
+                             int64_t min_size_for_strassen, const uint8_t* tree_bits,
+                             const uint8_t* dense_bits, int32_t* leaf_tasks, int64_t cap,
+                             int64_t* nleaf, int64_t* nroots, int64_t* ndense) {
   if (!uniform_square(A, B) || !same_geometry(C, A)) return -2;
   const int64_t nib = A->nrb;
   const int64_t bs = A->rscr[0];
@@ -791,7 +820,7 @@ int or_matmult_strassen_tree(or_bsm* C, const or_bsm* A, const or_bsm* B, double
 
   hctx_t h;
   memset(&h, 0, sizeof(h));
-  h.C = C; h.A = A; h.B = B; h.bits = bits; h.lvl_off = lvl_off; h.thr = thr;
+  h.C = C; h.A = A; h.B = B; h.bits = bits; h.dense = dense_bits; h.lvl_off = lvl_off; h.thr = thr;
   h.sc.bs = bs; h.sc.cutoff = min_size_for_strassen;
   h.leaf = leaf_tasks; h.cap = cap;
   grid_t ag = {A->blk, A->nrb, A->ncb, A->nrb}, bg = {B->blk, B->nrb, B->ncb, B->nrb}, cg = {C->blk, C->nrb, C->ncb, C->nrb};
@@ -805,6 +834,7 @@ int or_matmult_strassen_tree(or_bsm* C, const or_bsm* A, const or_bsm* B, double
   if (leaf_tasks) qsort(leaf_tasks, (size_t)(h.nleaf < cap ? h.nleaf : cap), 3 * sizeof(int32_t), cmp_task);
   if (nleaf) *nleaf = h.nleaf;
   if (nroots) *nroots = h.nroots;
+  if (ndense) *ndense = h.ndense;
   free(lvl_off); free(bits); free(lsz);
   return 0;
 }

@@ -115,6 +115,18 @@ int or_matmult_strassen_tree(or_bsm* C, const or_bsm* A, const or_bsm* B, double
                              int64_t min_size_for_strassen, const uint8_t* tree_bits,
                              int32_t* leaf_tasks, int64_t cap, int64_t* nleaf, int64_t* nroots);
 
+/* As or_matmult_strassen_tree, plus dense roots: a node whose dense_bits
+ * entry is set (and whose Strassen bit is not) adds every product of present
+ * blocks inside its region, without screening, into the present C blocks of
+ * the region (C blocks visited column-major, inner blocks ascending).  This is
+ * the "dense" path of the learned path selector; it has no counterpart in the
+ * reference, whose dense fallback (strassen.hpp:296-353) exists only inside
+ * Strassen nodes.  ndense (optional) gets the number of dense roots. */
+int or_matmult_strassen_plan(or_bsm* C, const or_bsm* A, const or_bsm* B, double thresh,
+                             int64_t min_size_for_strassen, const uint8_t* tree_bits,
+                             const uint8_t* dense_bits, int32_t* leaf_tasks, int64_t cap,
+                             int64_t* nleaf, int64_t* nroots, int64_t* ndense);
+
 /* Dense column-major GEMM used by the oracle's block products:
  * C = alpha op(A) op(B) + beta C (beta==0: C not read). */
 void or_dgemm(int transA, int transB, int64_t m, int64_t n, int64_t k, double alpha,

@@ -47,7 +47,7 @@ INT = C.c_int
 
 class Opts(C.Structure):
     _fields_ = [("mode", I32), ("slices", I32), ("slice_type", I32), ("tile", I32),
-                ("scheduler", I32), ("reserved", I32), ("cutoff", I64), ("tree", P)]
+                ("scheduler", I32), ("reserved", I32), ("cutoff", I64), ("tree", P), ("tree_dense", P)]
 
 
 # name -> (restype, argtypes); every function declared in include/strassen_amd.h
@@ -63,6 +63,16 @@ ABI = {
     "sa_ctx_profile_reset": (INT, [P]),
     "sa_ctx_gemm_config": (INT, [P, P, C.c_char_p, I32]),
     "sa_ctx_kwindows": (INT, [P, P]),
+    "sa_mlp_info": (INT, [P, P, P]),
+    "sa_mlp_set": (INT, [P, P, I64, P, P]),
+    "sa_mlp_get": (INT, [P, P, P, P]),
+    "sa_mlp_load": (INT, [P, C.c_char_p]),
+    "sa_mlp_forward": (INT, [P, P, I64, P]),
+    "sa_mlp_grad": (INT, [P, P, P, P, I64, P, P]),
+    "sa_mlp_train": (INT, [P, P, P, P, I64, I32, D, P]),
+    "sa_strassen_features": (INT, [P, P, P, D, P, I64, P]),
+    "sa_strassen_plan": (INT, [P, P, P, D, P, P, I64, P, P, P]),
+    "sa_strassen_last_plan": (INT, [P, P, P, P]),
     "sa_ctx_mark": (INT, [P, INT]),
     "sa_ctx_elapsed": (INT, [P, INT, INT, P]),
     "sa_bsm_create": (INT, [P, I64, I64, I32, I32, D, P]),
@@ -94,7 +104,7 @@ ABI = {
     "sa_ri_k_step": (INT, [P, P, P, P, I32, P, I32, P, D, P, P]),
     "sa_strassen_dryrun": (INT, [P, P, D, P, I64, P, P, P]),
     "sa_matmult_strassen_sparse": (INT, [P, P, INT, P, INT, D, D, D, P]),
-    "sa_strassen_last_stats": (INT, [P, P, P, P, P, P]),
+    "sa_strassen_last_stats": (INT, [P, P, P, P, P, P, P]),
 }
 
 
@@ -137,10 +147,10 @@ def default_opts(**kw) -> Opts:
     o = Opts()
     lib().sa_opts_default(C.byref(o))
     for k, v in kw.items():
-        if k == "tree":
+        if k in ("tree", "tree_dense"):
             arr = np.ascontiguousarray(np.concatenate(v) if isinstance(v, (list, tuple)) else v, dtype=np.uint8)
-            o._tree_keep = arr
-            o.tree = arr.ctypes.data
+            setattr(o, "_" + k + "_keep", arr)
+            setattr(o, k, arr.ctypes.data)
         else:
             setattr(o, k, v)
     return o
@@ -357,21 +367,137 @@ def strassen_dryrun(A: BSM, B: BSM, thresh_per_block):
     return out, lfl
 
 
-def matmult_strassen_sparse(Cm: BSM, A: BSM, B: BSM, thresh, cutoff=0, opts: Opts | None = None, tree=None):
+def matmult_strassen_sparse(Cm: BSM, A: BSM, B: BSM, thresh, cutoff=0, opts: Opts | None = None, tree=None,
+                            dense=None):
+    """Hybrid Strassen-sparse multiply (strassen.hpp:687-753).  tree / dense:
+    caller Strassen-root / dense-root bits (per-level lists, dry-run layout)."""
     o = opts if opts is not None else default_opts()
     o.cutoff = int(cutoff)
-    if tree is not None:
-        arr = np.ascontiguousarray(np.concatenate(tree) if len(tree) else np.zeros(1), dtype=np.uint8)
-        o._tree_keep = arr
-        o.tree = arr.ctypes.data
-        o.scheduler = SCHED_TREE
+    for name, bits in (("tree", tree), ("tree_dense", dense)):
+        if bits is not None:
+            arr = np.ascontiguousarray(np.concatenate(bits) if len(bits) else np.zeros(1), dtype=np.uint8)
+            setattr(o, "_" + name + "_keep", arr)
+            setattr(o, name, arr.ctypes.data)
+            o.scheduler = SCHED_TREE
     _check(lib().sa_matmult_strassen_sparse(Cm.h, A.h, 0, B.h, 0, thresh, 1.0, 0.0, C.byref(o)))
 
 
+# ------------------------------------------------------ learned path selector --
+DEFAULT_PATH_MODEL = os.path.join(os.path.dirname(__file__), "data", "path_mlp.txt")
+
+
+def mlp_info():
+    """(features, outputs, parameters) of the path-selector network."""
+    f, o, p = I32(0), I32(0), I64(0)
+    _check(lib().sa_mlp_info(C.byref(f), C.byref(o), C.byref(p)))
+    return f.value, o.value, p.value
+
+
+def _rows(x, ncol):
+    return np.ascontiguousarray(np.asarray(x, dtype=np.float64).reshape(-1, ncol))
+
+
+def mlp_set(ctx: Context, params, mu=None, sd=None):
+    f, _, npar = mlp_info()
+    prm = np.ascontiguousarray(params, dtype=np.float64)
+    assert prm.size == npar
+    mu = np.ascontiguousarray(np.zeros(f) if mu is None else mu, dtype=np.float64)
+    sd = np.ascontiguousarray(np.ones(f) if sd is None else sd, dtype=np.float64)
+    _check(lib().sa_mlp_set(ctx.h, _ptr(prm), npar, _ptr(mu), _ptr(sd)))
+
+
+def mlp_get(ctx: Context):
+    f, _, npar = mlp_info()
+    prm, mu, sd = np.zeros(npar), np.zeros(f), np.zeros(f)
+    _check(lib().sa_mlp_get(ctx.h, _ptr(prm), _ptr(mu), _ptr(sd)))
+    return prm, mu, sd
+
+
+def mlp_load(ctx: Context, path: str = DEFAULT_PATH_MODEL):
+    _check(lib().sa_mlp_load(ctx.h, path.encode()))
+
+
+def mlp_save(ctx: Context, path: str):
+    f, o, npar = mlp_info()
+    prm, mu, sd = mlp_get(ctx)
+    with open(path, "w") as fh:
+        fh.write(f"path_mlp {f} 32 {o} {npar}\n")
+        for v in np.concatenate([prm, mu, sd]):
+            fh.write(repr(float(v)) + "\n")
+
+
+def mlp_forward(ctx: Context, X):
+    f, o, _ = mlp_info()
+    X = _rows(X, f)
+    Y = np.zeros((X.shape[0], o))
+    _check(lib().sa_mlp_forward(ctx.h, _ptr(X), X.shape[0], _ptr(Y)))
+    return Y
+
+
+def mlp_grad(ctx: Context, X, T, M=None):
+    f, o, npar = mlp_info()
+    X, T = _rows(X, f), _rows(T, o)
+    M = None if M is None else _rows(M, o)
+    g = np.zeros(npar)
+    loss = D(0.0)
+    _check(lib().sa_mlp_grad(ctx.h, _ptr(X), _ptr(T), None if M is None else _ptr(M), X.shape[0], _ptr(g),
+                             C.byref(loss)))
+    return g, loss.value
+
+
+def mlp_train(ctx: Context, X, T, M=None, epochs=1000, lr=3e-3):
+    f, o, _ = mlp_info()
+    X, T = _rows(X, f), _rows(T, o)
+    M = None if M is None else _rows(M, o)
+    loss = D(0.0)
+    _check(lib().sa_mlp_train(ctx.h, _ptr(X), _ptr(T), None if M is None else _ptr(M), X.shape[0], int(epochs),
+                              float(lr), C.byref(loss)))
+    return loss.value
+
+
+def strassen_features(Cm: BSM | None, A: BSM, B: BSM, thresh):
+    """Node features of every dry-run node (rows in dry-run order)."""
+    f, _, _ = mlp_info()
+    nn = I64(0)
+    _check(lib().sa_strassen_features(None if Cm is None else Cm.h, A.h, B.h, thresh, None, 0, C.byref(nn)))
+    X = np.zeros((max(nn.value, 1), f))
+    _check(lib().sa_strassen_features(None if Cm is None else Cm.h, A.h, B.h, thresh, _ptr(X), X.size,
+                                      C.byref(nn)))
+    return X[: nn.value]
+
+
+def _split_levels(flat, nb):
+    out, off = [], 0
+    for s in tree_level_sizes(nb):
+        out.append(flat[off:off + s].copy())
+        off += s
+    return out
+
+
+def strassen_plan(Cm: BSM | None, A: BSM, B: BSM, thresh):
+    """Learned plan: (Strassen bits, dense bits) per level, overruled flag,
+    model flops of the plan and of the heuristic plan."""
+    total = max(sum(tree_level_sizes(A.nrb)), 1)
+    sb = np.zeros(total, dtype=np.uint8)
+    db = np.zeros(total, dtype=np.uint8)
+    ov = I32(0)
+    mc, hc = D(0.0), D(0.0)
+    _check(lib().sa_strassen_plan(None if Cm is None else Cm.h, A.h, B.h, thresh, _ptr(sb), _ptr(db), total,
+                                  C.byref(ov), C.byref(mc), C.byref(hc)))
+    return _split_levels(sb, A.nrb), _split_levels(db, A.nrb), bool(ov.value), mc.value, hc.value
+
+
+def strassen_last_plan():
+    s, ov = I32(0), I32(0)
+    mc, hc = D(0.0), D(0.0)
+    _check(lib().sa_strassen_last_plan(C.byref(s), C.byref(ov), C.byref(mc), C.byref(hc)))
+    return {"scheduler": s.value, "overruled": bool(ov.value), "model_cost": mc.value, "heuristic_cost": hc.value}
+
+
 def strassen_last_stats():
-    v = [I64(0) for _ in range(6)]
+    v = [I64(0) for _ in range(7)]
     _check(lib().sa_strassen_last_stats(*[C.byref(x) for x in v]))
-    keys = ["nroots", "nnodes", "nleaf_products", "scratch_bytes", "batches", "screened"]
+    keys = ["nroots", "nnodes", "nleaf_products", "scratch_bytes", "batches", "screened", "dense_roots"]
     return {k: x.value for k, x in zip(keys, v)}
 
 

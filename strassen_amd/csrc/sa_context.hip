@@ -207,6 +207,7 @@ void sa_opts_default(sa_opts_t* o) {
   o->scheduler = SA_SCHED_HEURISTIC;
   o->cutoff = 0;
   o->tree = nullptr;
+  o->tree_dense = nullptr;
 }
 
 const char* sa_last_error(void) { return g_err; }

@@ -144,6 +144,9 @@ struct sa_context {
   int32_t last_nib = 0;
   int32_t last_gemm_cfg = -1;
   int32_t last_kwindows = 1;  // k-windows of the last screened multiply
+  // learned path selector (device copy of the network in buffer "mlp_prm")
+  bool mlp_loaded = false;
+  int32_t mlp_step = 0;  // Adam steps taken
   sa::DevBuf& buf(const char* name) { return bufs[name]; }
   // profiling
   bool profiling = false;
@@ -280,6 +283,16 @@ struct KWindows {
 // nwin == 1 (and outs == full records) when windows do not pay.
 KWindows plan_kwindows(sa_context* c, sa_bsm* A, bool ta, sa_bsm* B, bool tb, const ScreenResult& R,
                        const GemmOut* outs);
+// learned path selector network (sa_mlp.hip)
+constexpr int kMlpFeatures = 10, kMlpOutputs = 2;
+constexpr int kMlpParams = 32 * kMlpFeatures + 32 + 32 * 32 + 32 + kMlpOutputs * 32 + kMlpOutputs;
+bool mlp_loaded(sa_context* c);
+// Y (n x kMlpOutputs) = network(X (n x kMlpFeatures)), device arrays, async
+void mlp_forward_device(sa_context* c, const double* dX, int64_t n, double* dY);
+// rocBLAS on the context stream: C_i = A_i B_i (column-major, no transposes)
+void device_dgemm_strided_batched(sa_context* c, int64_t m, int64_t n, int64_t k, const double* A, int64_t lda,
+                                  int64_t sA, const double* B, int64_t ldb, int64_t sB, double* Cm, int64_t ldc,
+                                  int64_t sC, int64_t count);
 // Tile configurations of the FP64 MFMA kernel.
 int gemm_fp64_num_configs();
 const char* gemm_fp64_config_name(int cfg);

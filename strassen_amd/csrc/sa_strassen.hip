@@ -387,20 +387,26 @@ __global__ void k_leaf_records(const LeafProd* __restrict__ lp, Ptrs P, int32_t 
 struct Root {
   int level;
   int64_t ib0, jb0, kb0;
+  int kind;  // 0 Strassen-Winograd, 1 dense
 };
 
-static void find_roots(const Tree& t, const std::vector<uint8_t>& bits, int32_t nb, std::vector<Root>& roots) {
+// topmost nodes whose Strassen bit (else dense bit, when dbits is given) is set
+static void find_roots(const Tree& t, const std::vector<uint8_t>& bits, const std::vector<uint8_t>& dbits,
+                       int32_t nb, std::vector<Root>& roots) {
   if (t.nlev == 0) return;
   struct F {
     const Tree& t;
     const std::vector<uint8_t>& bits;
+    const std::vector<uint8_t>& dbits;
     int32_t nb;
     std::vector<Root>& roots;
     void visit(int l, int64_t si, int64_t sj, int64_t sk) {
       const int64_t ns = ceil_div(nb, (int64_t)1 << (l + 1));
-      if (bits[t.lvl_off[l] + si + sj * ns + sk * ns * ns]) {
+      const int64_t node = t.lvl_off[l] + si + sj * ns + sk * ns * ns;
+      const bool st = bits[node] != 0, dn = !dbits.empty() && dbits[node] != 0;
+      if (st || dn) {
         const int64_t sb = (int64_t)1 << (l + 1);
-        roots.push_back(Root{l, si * sb, sj * sb, sk * sb});
+        roots.push_back(Root{l, si * sb, sj * sb, sk * sb, st ? 0 : 1});
         return;
       }
       if (l == 0) return;
@@ -419,7 +425,7 @@ static void find_roots(const Tree& t, const std::vector<uint8_t>& bits, int32_t 
         }
       }
     }
-  } f{t, bits, nb, roots};
+  } f{t, bits, dbits, nb, roots};
   const int top = t.nlev - 1;
   const int64_t ntop = ceil_div(nb, (int64_t)1 << t.nlev);
   for (int64_t sk = 0; sk < ntop; ++sk)
@@ -617,8 +623,380 @@ static void run_plan(sa_bsm* C, sa_bsm* A, sa_bsm* B, Plan& p, int tile_cfg) {
   SA_CHECK_HIP(hipStreamSynchronize(c->stream));  // host item vectors go out of scope
 }
 
+// ------------------------------------------------------------- dense roots --
+// A dense root at level l covers w = 2^(l+1) blocks per side: its A region
+// (rows ib0.., columns kb0..) and B region (rows kb0.., columns jb0..) are
+// packed column-major into W x W arrays (W = w bs, absent blocks as zeros),
+// multiplied by one strided-batched library GEMM per batch of equal-level roots,
+// and the W x W products are added into the present C blocks of the region.
+// Roots sharing a C region are added in ascending kb0 order, one launch per
+// rank, so every C block sums its contributions in a fixed order.
+__global__ void k_pack_regions(const int64_t* __restrict__ off, const double* __restrict__ arena, int32_t nrb,
+                               const int32_t* __restrict__ r0, const int32_t* __restrict__ c0, int32_t w, int32_t bs,
+                               double* __restrict__ dst) {
+  const int64_t g = blockIdx.x;
+  const int64_t ww = (int64_t)w * w;
+  const int64_t root = g / ww;
+  const int rem = (int)(g % ww);
+  const int bi = rem % w, bj = rem / w;
+  const int64_t W = (int64_t)w * bs;
+  double* d = dst + root * W * W + (int64_t)bj * bs * W + (int64_t)bi * bs;
+  const int64_t o = off[(int64_t)(r0[root] + bi) + (int64_t)(c0[root] + bj) * nrb];
+  for (int e = threadIdx.x; e < bs * bs; e += blockDim.x) {
+    const int r = e % bs, cc = e / bs;
+    d[(int64_t)cc * W + r] = o >= 0 ? arena[o + e] : 0.0;
+  }
+}
+
+__global__ void k_scatter_regions(const int64_t* __restrict__ coff, double* __restrict__ carena, int32_t nrb,
+                                  const int32_t* __restrict__ r0, const int32_t* __restrict__ c0,
+                                  const int32_t* __restrict__ sel, int32_t w, int32_t bs,
+                                  const double* __restrict__ src) {
+  const int64_t g = blockIdx.x;
+  const int64_t ww = (int64_t)w * w;
+  const int32_t root = sel[g / ww];
+  const int rem = (int)(g % ww);
+  const int bi = rem % w, bj = rem / w;
+  const int64_t o = coff[(int64_t)(r0[root] + bi) + (int64_t)(c0[root] + bj) * nrb];
+  if (o < 0) return;  // += into an empty block is dropped, as for Strassen roots
+  const int64_t W = (int64_t)w * bs;
+  const double* sp = src + (int64_t)root * W * W + (int64_t)bj * bs * W + (int64_t)bi * bs;
+  for (int e = threadIdx.x; e < bs * bs; e += blockDim.x) {
+    const int r = e % bs, cc = e / bs;
+    carena[o + e] += sp[(int64_t)cc * W + r];
+  }
+}
+
+// dense roots of one level; returns the number executed
+static int64_t run_dense_level(sa_bsm* C, sa_bsm* A, sa_bsm* B, std::vector<Root> rs) {
+  if (rs.empty()) return 0;
+  sa_context* c = C->ctx;
+  const int32_t bs = A->bs_r;
+  const int32_t w = 1 << (rs[0].level + 1);
+  const int64_t W = (int64_t)w * bs, W2 = W * W;
+  std::stable_sort(rs.begin(), rs.end(), [](const Root& x, const Root& y) {
+    if (x.jb0 != y.jb0) return x.jb0 < y.jb0;
+    if (x.ib0 != y.ib0) return x.ib0 < y.ib0;
+    return x.kb0 < y.kb0;
+  });
+  size_t free_b = 0, total_b = 0;
+  SA_CHECK_HIP(hipMemGetInfo(&free_b, &total_b));
+  const int64_t per = 3 * W2 * (int64_t)sizeof(double);
+  const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>((int64_t)(free_b / 4) / per, 4096));
+  DevBuf& bp = c->buf("dn_pack");
+  bp.reserve((size_t)(std::min<int64_t>(chunk, (int64_t)rs.size()) * per) + 64);
+  ProfScope ps(c, "dense_roots");
+  for (size_t i0 = 0; i0 < rs.size(); i0 += (size_t)chunk) {
+    const size_t n = std::min(rs.size() - i0, (size_t)chunk);
+    std::vector<int32_t> ai(n), ak(n), bk(n), bj(n);
+    std::vector<int32_t> ci(n), cj(n);
+    for (size_t q = 0; q < n; ++q) {
+      const Root& r = rs[i0 + q];
+      ai[q] = (int32_t)r.ib0; ak[q] = (int32_t)r.kb0;
+      bk[q] = (int32_t)r.kb0; bj[q] = (int32_t)r.jb0;
+      ci[q] = (int32_t)r.ib0; cj[q] = (int32_t)r.jb0;
+    }
+    // ranks among roots of one C region (sorted by kb0 within it)
+    std::vector<std::vector<int32_t>> ranks;
+    for (size_t q = 0; q < n; ++q) {
+      size_t rk = 0;
+      if (q > 0) {
+        size_t p = q;
+        while (p > 0 && ci[p - 1] == ci[q] && cj[p - 1] == cj[q]) --p;
+        rk = q - p;
+      }
+      if (ranks.size() <= rk) ranks.resize(rk + 1);
+      ranks[rk].push_back((int32_t)q);
+    }
+    double* PA = bp.as<double>();
+    double* PB = PA + (int64_t)n * W2;
+    double* PC = PB + (int64_t)n * W2;
+    const int32_t* dai = upload(c, "dn_ai", ai);
+    const int32_t* dak = upload(c, "dn_ak", ak);
+    const int32_t* dbk = upload(c, "dn_bk", bk);
+    const int32_t* dbj = upload(c, "dn_bj", bj);
+    const unsigned nblk = (unsigned)(n * (size_t)w * (size_t)w);
+    hipLaunchKernelGGL(k_pack_regions, dim3(nblk), dim3(256), 0, c->stream, (const int64_t*)A->d_off,
+                       (const double*)A->data(), A->nrb, dai, dak, w, bs, PA);
+    hipLaunchKernelGGL(k_pack_regions, dim3(nblk), dim3(256), 0, c->stream, (const int64_t*)B->d_off,
+                       (const double*)B->data(), B->nrb, dbk, dbj, w, bs, PB);
+    device_dgemm_strided_batched(c, W, W, W, PA, W, W2, PB, W, W2, PC, W, W2, (int64_t)n);
+    for (size_t rk = 0; rk < ranks.size(); ++rk) {
+      std::string nm = "dn_sel" + std::to_string(rk);
+      const int32_t* dsel = upload(c, nm.c_str(), ranks[rk]);
+      hipLaunchKernelGGL(k_scatter_regions, dim3((unsigned)(ranks[rk].size() * (size_t)w * (size_t)w)), dim3(256), 0,
+                         c->stream, (const int64_t*)C->d_off, C->data(), C->nrb, dai, dbj, dsel, w, bs,
+                         (const double*)PC);
+    }
+    SA_CHECK_HIP(hipGetLastError());
+    SA_CHECK_HIP(hipStreamSynchronize(c->stream));  // host index vectors go out of scope
+  }
+  return (int64_t)rs.size();
+}
+
+// ------------------------------------------------------- learned planning --
+// Strassen node cost of the reference model at level l (strassen.hpp:590-600)
+static uint64_t strassen_model_cost(int l, uint64_t bs) {
+  const uint64_t cost_mult = (uint64_t)std::round(std::pow(7.0, (double)(l + 1))) * (2u * bs * bs * bs);
+  const uint64_t cost_add = (uint64_t)(15e0 * kStrassenFac[l] * std::pow(4.0, (double)(l + 1)) * (double)(bs * bs));
+  return cost_mult + cost_add;
+}
+
+// summed-area table of block presence: sat[r + c (nb+1)] = #present blocks with row < r, col < c
+__device__ inline int32_t sat_count(const int32_t* sat, int32_t nb, int64_t r0, int64_t r1, int64_t c0, int64_t c1) {
+  const int64_t ld = nb + 1;
+  return sat[r1 + c1 * ld] - sat[r0 + c1 * ld] - sat[r1 + c0 * ld] + sat[r0 + c0 * ld];
+}
+
+// screened flops of every level-0 node (the dry run's count before any decision)
+__global__ void k_raw_level0(DryArgs a, int64_t ns, double* __restrict__ raw) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns * ns * ns) return;
+  const int64_t si = s % ns, sj = (s / ns) % ns, sk = s / (ns * ns);
+  double f = 0.0;
+  for (int64_t di = 0; di < 2; ++di)
+    for (int64_t dj = 0; dj < 2; ++dj)
+      for (int64_t dk = 0; dk < 2; ++dk) {
+        const int64_t ib = 2 * si + di, jb = 2 * sj + dj, kb = 2 * sk + dk;
+        if (ib >= a.nb || jb >= a.nb || kb >= a.nb) continue;
+        const int64_t ai = ib + kb * a.nb, bi = kb + jb * a.nb;
+        if (a.aoff[ai] < 0 || a.boff[bi] < 0) continue;
+        if (a.anorm[ai] * a.bnorm[bi] >= a.thr) f += 2.0 * a.rsz[ib] * a.rsz[jb] * (double)a.rsz[kb];
+      }
+  raw[s] = f;
+}
+
+__global__ void k_raw_level(int32_t nb, int level, int64_t ns, int64_t pns, const double* __restrict__ prev,
+                            double* __restrict__ cur) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns * ns * ns) return;
+  const int64_t si = s % ns, sj = (s / ns) % ns, sk = s / (ns * ns);
+  const int64_t psb = (int64_t)1 << level, sb = psb * 2;
+  const int64_t ib0 = si * sb, jb0 = sj * sb, kb0 = sk * sb;
+  const int64_t isub = ceil_div(std::min<int64_t>(ib0 + sb, nb) - ib0, psb);
+  const int64_t jsub = ceil_div(std::min<int64_t>(jb0 + sb, nb) - jb0, psb);
+  const int64_t ksub = ceil_div(std::min<int64_t>(kb0 + sb, nb) - kb0, psb);
+  double f = 0.0;
+  for (int64_t x = 0; x < isub; ++x)
+    for (int64_t y = 0; y < jsub; ++y)
+      for (int64_t z = 0; z < ksub; ++z) f += prev[(2 * si + x) + (2 * sj + y) * pns + (2 * sk + z) * pns * pns];
+  cur[s] = f;
+}
+
+struct FeatArgs {
+  const int32_t *satA, *satB, *satC;
+  int32_t nb;
+  int64_t n;
+  int32_t bs;
+};
+
+__device__ inline bool node_edge(int32_t nb, int64_t n, int32_t bs, int64_t sb, int64_t ib0, int64_t jb0,
+                                 int64_t kb0) {
+  const int64_t ib1 = std::min<int64_t>(ib0 + sb, nb), jb1 = std::min<int64_t>(jb0 + sb, nb),
+                kb1 = std::min<int64_t>(kb0 + sb, nb);
+  const int64_t ie = std::min<int64_t>(ib1 * bs, n), je = std::min<int64_t>(jb1 * bs, n),
+                ke = std::min<int64_t>(kb1 * bs, n);
+  return (ie - ib0 * bs != bs * sb) || (je - jb0 * bs != bs * sb) || (ke - kb0 * bs != bs * sb);
+}
+
+// The selector's node features (all scaled to about [0, 1]):
+//  0 level/8, 1 log2(bs)/8, 2 log2(node side)/16, 3-4 block fill of the node's
+//  A and B regions, 5 screened flops / dense flops of the region, 6 block fill
+//  of its C region, 7 the cost-model (heuristic) decision, 8 edge node,
+//  9 log2(1 + screened flops)/64
+__global__ void k_node_features(FeatArgs fa, int level, int64_t ns, const double* __restrict__ raw,
+                                const uint8_t* __restrict__ hbits, double* __restrict__ X) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns * ns * ns) return;
+  const int64_t si = s % ns, sj = (s / ns) % ns, sk = s / (ns * ns);
+  const int64_t w = (int64_t)1 << (level + 1);
+  const int64_t ib0 = si * w, jb0 = sj * w, kb0 = sk * w;
+  const int64_t ib1 = std::min<int64_t>(ib0 + w, fa.nb), jb1 = std::min<int64_t>(jb0 + w, fa.nb),
+                kb1 = std::min<int64_t>(kb0 + w, fa.nb);
+  const double ww = (double)(w * w);
+  const double W = (double)w * fa.bs;
+  double* x = X + s * kMlpFeatures;
+  x[0] = level / 8.0;
+  x[1] = log2((double)fa.bs) / 8.0;
+  x[2] = log2(W) / 16.0;
+  x[3] = sat_count(fa.satA, fa.nb, ib0, ib1, kb0, kb1) / ww;
+  x[4] = sat_count(fa.satB, fa.nb, kb0, kb1, jb0, jb1) / ww;
+  x[5] = raw[s] / (2.0 * W * W * W);
+  x[6] = sat_count(fa.satC, fa.nb, ib0, ib1, jb0, jb1) / ww;
+  x[7] = hbits[s] ? 1.0 : 0.0;
+  x[8] = node_edge(fa.nb, fa.n, fa.bs, w, ib0, jb0, kb0) ? 1.0 : 0.0;
+  x[9] = log2(1.0 + raw[s]) / 64.0;
+}
+
+// Bottom-up plan: best(node) = min(descend, Strassen, dense) in units of
+// screened-path flops; descend = screened flops at level 0, else the sum of the
+// children's best; Strassen / dense = screened flops x 2^(predicted log2
+// ratio), interior nodes only.  Ties keep the descend choice.  The heuristic
+// judges every learned choice: a path is admissible only where the reference
+// cost model prices it (Strassen: its node cost; dense: the region's dense
+// flops) at no more than kJudge x the node's screened flops.
+constexpr double kJudge = 1.5;
+__global__ void k_plan_level(int32_t nb, int64_t n, int32_t bs, int level, int64_t ns, int64_t pns,
+                             double cost_s, const double* __restrict__ raw, const double* __restrict__ Y,
+                             const double* __restrict__ prev, double* __restrict__ best, uint8_t* __restrict__ sbit,
+                             uint8_t* __restrict__ dbit) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns * ns * ns) return;
+  const int64_t si = s % ns, sj = (s / ns) % ns, sk = s / (ns * ns);
+  const int64_t w = (int64_t)1 << (level + 1), psb = w / 2;
+  double desc = 0.0;
+  if (level == 0) {
+    desc = raw[s];
+  } else {
+    const int64_t ib0 = si * w, jb0 = sj * w, kb0 = sk * w;
+    const int64_t isub = ceil_div(std::min<int64_t>(ib0 + w, nb) - ib0, psb);
+    const int64_t jsub = ceil_div(std::min<int64_t>(jb0 + w, nb) - jb0, psb);
+    const int64_t ksub = ceil_div(std::min<int64_t>(kb0 + w, nb) - kb0, psb);
+    for (int64_t x = 0; x < isub; ++x)
+      for (int64_t y = 0; y < jsub; ++y)
+        for (int64_t z = 0; z < ksub; ++z) desc += prev[(2 * si + x) + (2 * sj + y) * pns + (2 * sk + z) * pns * pns];
+  }
+  double b = desc;
+  uint8_t sv = 0, dv = 0;
+  if (!node_edge(nb, n, bs, w, si * w, sj * w, sk * w)) {
+    const double W = (double)w * bs;
+    const double cs = raw[s] * exp2(Y[s * kMlpOutputs + 0]);
+    const double cd = raw[s] * exp2(Y[s * kMlpOutputs + 1]);
+    if (cs < b && cost_s <= kJudge * raw[s]) { b = cs; sv = 1; dv = 0; }
+    if (cd < b && 2.0 * W * W * W <= kJudge * raw[s]) { b = cd; sv = 0; dv = 1; }
+  }
+  best[s] = b;
+  sbit[s] = sv;
+  dbit[s] = dv;
+}
+
+struct MlpPlan {
+  bool used = false, overruled = false;
+  double model_cost = 0.0, heur_cost = 0.0;  // reference flop model of both plans
+};
+
+static std::vector<int32_t> presence_sat(sa_context* c, const sa_bsm* X) {
+  const int32_t nb = X->nrb;
+  std::vector<int64_t> off((size_t)X->nb());
+  if (!off.empty())
+    SA_CHECK_HIP(hipMemcpyAsync(off.data(), X->d_off, sizeof(int64_t) * off.size(), hipMemcpyDeviceToHost, c->stream));
+  SA_CHECK_HIP(hipStreamSynchronize(c->stream));
+  const int64_t ld = nb + 1;
+  std::vector<int32_t> sat((size_t)(ld * ld), 0);
+  for (int64_t cc = 1; cc <= nb; ++cc)
+    for (int64_t r = 1; r <= nb; ++r)
+      sat[r + cc * ld] = (off[(r - 1) + (cc - 1) * nb] >= 0 ? 1 : 0) + sat[(r - 1) + cc * ld] + sat[r + (cc - 1) * ld] -
+                         sat[(r - 1) + (cc - 1) * ld];
+  return sat;
+}
+
+// Features of every dry-run node (rows in dry-run order) into buffer "pl_x";
+// screened flops per node into "pl_raw".  Needs the dry run's bits in "st_bits".
+static void node_features(sa_bsm* C, sa_bsm* A, sa_bsm* B, double thr, const Tree& t) {
+  sa_context* c = A->ctx;
+  const int32_t nb = A->nrb;
+  std::vector<int32_t> sa_ = presence_sat(c, A), sb_ = presence_sat(c, B);
+  std::vector<int32_t> sc_;
+  if (C) {
+    sc_ = presence_sat(c, C);
+  } else {  // no C: every block counts as present
+    const int64_t ld = nb + 1;
+    sc_.assign((size_t)(ld * ld), 0);
+    for (int64_t cc = 0; cc <= nb; ++cc)
+      for (int64_t r = 0; r <= nb; ++r) sc_[r + cc * ld] = (int32_t)(r * cc);
+  }
+  const int32_t* dsa = upload(c, "pl_sat_a", sa_);
+  const int32_t* dsb = upload(c, "pl_sat_b", sb_);
+  const int32_t* dsc = upload(c, "pl_sat_c", sc_);
+  DevBuf& braw = c->buf("pl_raw");
+  DevBuf& bx = c->buf("pl_x");
+  braw.reserve(sizeof(double) * (size_t)(t.total + 1));
+  bx.reserve(sizeof(double) * (size_t)(t.total * kMlpFeatures + 1));
+  double* raw = braw.as<double>();
+  DryArgs a{A->d_off, A->d_norm, B->d_off, B->d_norm, A->d_rsz, A->nrb, A->nrow, A->bs_r, thr};
+  FeatArgs fa{dsa, dsb, dsc, nb, A->nrow, A->bs_r};
+  const uint8_t* hb = c->buf("st_bits").as<uint8_t>();
+  for (int l = 0; l < t.nlev; ++l) {
+    const int64_t ns = ceil_div(nb, (int64_t)1 << (l + 1));
+    const int64_t nn = ns * ns * ns;
+    if (l == 0) {
+      hipLaunchKernelGGL(k_raw_level0, dim3((unsigned)ceil_div(nn, 256)), dim3(256), 0, c->stream, a, ns,
+                         raw + t.lvl_off[0]);
+    } else {
+      const int64_t pns = ceil_div(nb, (int64_t)1 << l);
+      hipLaunchKernelGGL(k_raw_level, dim3((unsigned)ceil_div(nn, 256)), dim3(256), 0, c->stream, nb, l, ns, pns,
+                         (const double*)(raw + t.lvl_off[l - 1]), raw + t.lvl_off[l]);
+    }
+    hipLaunchKernelGGL(k_node_features, dim3((unsigned)ceil_div(nn, 256)), dim3(256), 0, c->stream, fa, l, ns,
+                       (const double*)(raw + t.lvl_off[l]), hb + t.lvl_off[l],
+                       bx.as<double>() + t.lvl_off[l] * kMlpFeatures);
+  }
+  SA_CHECK_HIP(hipGetLastError());
+}
+
+// The learned plan (each choice admitted by the reference cost model, see
+// k_plan_level).  As a last guard the heuristic plan stands if the learned one
+// costs more than 4x the heuristic plan's model flops.
+static MlpPlan plan_mlp(sa_bsm* C, sa_bsm* A, sa_bsm* B, double thr, const Tree& t, std::vector<uint8_t>& sbits,
+                        std::vector<uint8_t>& dbits) {
+  sa_context* c = A->ctx;
+  MlpPlan info;
+  ProfScope ps(c, "mlp_plan");
+  node_features(C, A, B, thr, t);
+  DevBuf& by = c->buf("pl_y");
+  DevBuf& bbest = c->buf("pl_best");
+  DevBuf& bsb = c->buf("pl_sbits");
+  DevBuf& bdb = c->buf("pl_dbits");
+  by.reserve(sizeof(double) * (size_t)(t.total * kMlpOutputs + 1));
+  bbest.reserve(sizeof(double) * (size_t)(t.total + 1));
+  bsb.reserve((size_t)t.total + 16);
+  bdb.reserve((size_t)t.total + 16);
+  mlp_forward_device(c, c->buf("pl_x").as<double>(), t.total, by.as<double>());
+  const double* raw = c->buf("pl_raw").as<double>();
+  const int32_t nb = A->nrb;
+  for (int l = 0; l < t.nlev; ++l) {
+    const int64_t ns = ceil_div(nb, (int64_t)1 << (l + 1));
+    const int64_t nn = ns * ns * ns;
+    const int64_t pns = l > 0 ? ceil_div(nb, (int64_t)1 << l) : 0;
+    hipLaunchKernelGGL(k_plan_level, dim3((unsigned)ceil_div(nn, 256)), dim3(256), 0, c->stream, nb, A->nrow, A->bs_r,
+                       l, ns, pns, (double)strassen_model_cost(l, (uint64_t)A->bs_r), raw + t.lvl_off[l], (const double*)(by.as<double>() + t.lvl_off[l] * kMlpOutputs),
+                       l > 0 ? (const double*)(bbest.as<double>() + t.lvl_off[l - 1]) : (const double*)nullptr,
+                       bbest.as<double>() + t.lvl_off[l], bsb.as<uint8_t>() + t.lvl_off[l],
+                       bdb.as<uint8_t>() + t.lvl_off[l]);
+  }
+  SA_CHECK_HIP(hipGetLastError());
+  sbits.resize((size_t)t.total);
+  dbits.resize((size_t)t.total);
+  std::vector<double> hraw((size_t)t.total);
+  SA_CHECK_HIP(hipMemcpyAsync(sbits.data(), bsb.p, (size_t)t.total, hipMemcpyDeviceToHost, c->stream));
+  SA_CHECK_HIP(hipMemcpyAsync(dbits.data(), bdb.p, (size_t)t.total, hipMemcpyDeviceToHost, c->stream));
+  SA_CHECK_HIP(hipMemcpyAsync(hraw.data(), raw, sizeof(double) * (size_t)t.total, hipMemcpyDeviceToHost, c->stream));
+  SA_CHECK_HIP(hipStreamSynchronize(c->stream));
+  // judge: the reference cost model (strassen.hpp:579-673) on both plans
+  std::vector<Root> roots;
+  find_roots(t, sbits, dbits, nb, roots);
+  const uint64_t bs = (uint64_t)A->bs_r;
+  double covered = 0.0, model = 0.0;
+  for (const Root& r : roots) {
+    const int64_t w = (int64_t)1 << (r.level + 1);
+    const int64_t ns = ceil_div(nb, w);
+    const int64_t node = t.lvl_off[r.level] + r.ib0 / w + (r.jb0 / w) * ns + (r.kb0 / w) * ns * ns;
+    covered += hraw[(size_t)node];
+    const double W = (double)w * (double)bs;
+    model += r.kind == 0 ? (double)strassen_model_cost(r.level, bs) : 2.0 * W * W * W;
+  }
+  model += (double)t.lvl_flops[0] - covered;
+  info.used = true;
+  info.model_cost = model;
+  info.heur_cost = (double)t.lvl_flops[t.nlev];
+  info.overruled = model > 4.0 * info.heur_cost;
+  return info;
+}
+
 struct StrassenStats {
-  int64_t nroots = 0, nnodes = 0, nleaf_products = 0, scratch_bytes = 0, batches = 0, screened = 0;
+  int64_t nroots = 0, nnodes = 0, nleaf_products = 0, scratch_bytes = 0, batches = 0, screened = 0, dense = 0;
+  int32_t scheduler = SA_SCHED_HEURISTIC;  // the schedule that ran (MLP only when used and not overruled)
+  MlpPlan mlp;
 };
 
 static StrassenStats g_last_stats;
@@ -628,19 +1006,41 @@ static void hybrid(sa_bsm* C, sa_bsm* A, sa_bsm* B, double thresh, const sa_opts
   const int64_t bs = A->bs_r;
   const double thr = thresh * (double)(bs * bs * bs);
   Tree t = tree_shape(A->nrb);
-  std::vector<uint8_t> bits;
+  std::vector<uint8_t> bits, dbits;
+  MlpPlan mlp_info;
+  int32_t sched = o.scheduler == SA_SCHED_TREE ? SA_SCHED_TREE : SA_SCHED_HEURISTIC;
   if (o.scheduler == SA_SCHED_TREE) {
-    SA_REQUIRE(o.tree || t.total == 0, "strassen: scheduler TREE needs opts.tree");
-    bits.assign(o.tree, o.tree + t.total);
-    upload(c, "st_bits", bits);
-  } else if (o.scheduler == SA_SCHED_HEURISTIC) {
+    SA_REQUIRE(o.tree || o.tree_dense || t.total == 0, "strassen: scheduler TREE needs opts.tree or opts.tree_dense");
+    if (o.tree)
+      bits.assign(o.tree, o.tree + t.total);
+    else
+      bits.assign((size_t)t.total, 0);
+    if (o.tree_dense) dbits.assign(o.tree_dense, o.tree_dense + t.total);
+  } else if (o.scheduler == SA_SCHED_HEURISTIC || o.scheduler == SA_SCHED_MLP) {
     dryrun_device(A, B, thr, t);
     download_tree(c, t, bits);
+    // the learned selector, when a model is loaded; the heuristic otherwise,
+    // and whenever the reference cost model overrules the learned plan
+    if (o.scheduler == SA_SCHED_MLP && mlp_loaded(c) && t.nlev > 0) {
+      std::vector<uint8_t> sb, db;
+      mlp_info = plan_mlp(C, A, B, thr, t, sb, db);
+      if (!mlp_info.overruled) {
+        bits.swap(sb);
+        dbits.swap(db);
+        sched = SA_SCHED_MLP;
+      }
+    }
   } else {
     SA_UNSUPPORTED("strassen: scheduler %d not available", o.scheduler);
   }
   std::vector<Root> roots;
-  find_roots(t, bits, A->nrb, roots);
+  find_roots(t, bits, dbits, A->nrb, roots);
+  // screened leaves skip every triple under a Strassen or dense root
+  {
+    std::vector<uint8_t> cover(bits);
+    for (size_t q = 0; q < dbits.size(); ++q) cover[q] |= dbits[q];
+    upload(c, "st_bits", cover);
+  }
   // an executed root must cover full blocks only (never an edge super-block)
   for (const Root& r : roots) {
     const int64_t sb = (int64_t)1 << (r.level + 1);
@@ -649,8 +1049,16 @@ static void hybrid(sa_bsm* C, sa_bsm* A, sa_bsm* B, double thresh, const sa_opts
                "strassen: decision tree selects edge super-block (level %d at %lld,%lld,%lld)", r.level,
                (long long)r.ib0, (long long)r.jb0, (long long)r.kb0);
   }
+  std::vector<std::vector<Root>> dense_by_level(t.nlev > 0 ? t.nlev : 1);
+  {
+    std::vector<Root> sroots;
+    for (const Root& r : roots) (r.kind == 0 ? sroots : dense_by_level[r.level]).push_back(r);
+    roots.swap(sroots);
+  }
   StrassenStats st;
   st.nroots = (int64_t)roots.size();
+  st.scheduler = sched;
+  st.mlp = mlp_info;
   // screened leaves outside every executed root; C blocks are never created
   if (t.nlev > 0) {
     std::vector<int64_t> lo(t.lvl_off.begin(), t.lvl_off.end());
@@ -660,6 +1068,7 @@ static void hybrid(sa_bsm* C, sa_bsm* A, sa_bsm* B, double thresh, const sa_opts
   } else {
     st.screened = screened_multiply(C, A, false, B, false, thresh, 1.0, o.tile, nullptr, nullptr, 0, true);
   }
+  for (auto& lv : dense_by_level) st.dense += run_dense_level(C, A, B, lv);
   if (roots.empty()) {
     g_last_stats = st;
     return;
@@ -752,9 +1161,67 @@ int sa_matmult_strassen_sparse(sa_bsm_t C, sa_bsm_t A, int transA, sa_bsm_t B, i
   });
 }
 
-int sa_strassen_last_stats(int64_t* nroots, int64_t* nnodes, int64_t* nleaf_products, int64_t* scratch_bytes,
-                           int64_t* batches, int64_t* screened) {
+int sa_strassen_features(sa_bsm_t C, sa_bsm_t A, sa_bsm_t B, double thresh, double* feat, int64_t cap,
+                         int64_t* nnodes) {
   return guard([&] {
+    SA_REQUIRE(A && B && nnodes, "null argument");
+    SA_REQUIRE(uniform_square(A, B), "features: A and B must be square with equal uniform blocking");
+    SA_REQUIRE(!C || (C->rsz == A->rsz && C->csz == A->csz), "features: C geometry differs from A");
+    SA_CHECK_HIP(hipSetDevice(A->ctx->device));
+    const int64_t bs = A->bs_r;
+    const double thr = thresh * (double)(bs * bs * bs);
+    Tree t = tree_shape(A->nrb);
+    *nnodes = t.total;
+    if (t.total == 0) return;
+    dryrun_device(A, B, thr, t);
+    node_features(C, A, B, thr, t);
+    if (feat) {
+      SA_REQUIRE(cap >= t.total * kMlpFeatures, "features: capacity %lld < %lld", (long long)cap,
+                 (long long)(t.total * kMlpFeatures));
+      SA_CHECK_HIP(hipMemcpyAsync(feat, A->ctx->buf("pl_x").p, sizeof(double) * (size_t)(t.total * kMlpFeatures),
+                                  hipMemcpyDeviceToHost, A->ctx->stream));
+      SA_CHECK_HIP(hipStreamSynchronize(A->ctx->stream));
+    }
+  });
+}
+
+int sa_strassen_plan(sa_bsm_t C, sa_bsm_t A, sa_bsm_t B, double thresh, uint8_t* sbits, uint8_t* dbits, int64_t cap,
+                     int32_t* overruled, double* model_cost, double* heuristic_cost) {
+  return guard([&] {
+    SA_REQUIRE(A && B && sbits && dbits, "null argument");
+    SA_REQUIRE(uniform_square(A, B), "plan: A and B must be square with equal uniform blocking");
+    SA_REQUIRE(mlp_loaded(A->ctx), "plan: no path model loaded");
+    SA_CHECK_HIP(hipSetDevice(A->ctx->device));
+    const int64_t bs = A->bs_r;
+    const double thr = thresh * (double)(bs * bs * bs);
+    Tree t = tree_shape(A->nrb);
+    SA_REQUIRE(cap >= t.total, "plan: capacity %lld < %lld", (long long)cap, (long long)t.total);
+    if (t.total == 0) return;
+    std::vector<uint8_t> hb, sb, db;
+    dryrun_device(A, B, thr, t);
+    download_tree(A->ctx, t, hb);
+    const MlpPlan m = plan_mlp(C, A, B, thr, t, sb, db);
+    std::memcpy(sbits, sb.data(), (size_t)t.total);
+    std::memcpy(dbits, db.data(), (size_t)t.total);
+    if (overruled) *overruled = m.overruled ? 1 : 0;
+    if (model_cost) *model_cost = m.model_cost;
+    if (heuristic_cost) *heuristic_cost = m.heur_cost;
+  });
+}
+
+int sa_strassen_last_plan(int32_t* scheduler, int32_t* overruled, double* model_cost, double* heuristic_cost) {
+  return guard([&] {
+    if (scheduler) *scheduler = g_last_stats.scheduler;
+    if (overruled) *overruled = g_last_stats.mlp.overruled ? 1 : 0;
+    if (model_cost) *model_cost = g_last_stats.mlp.model_cost;
+    if (heuristic_cost) *heuristic_cost = g_last_stats.mlp.heur_cost;
+  });
+}
+
+int sa_strassen_last_stats(int64_t* nroots, int64_t* nnodes, int64_t* nleaf_products, int64_t* scratch_bytes,
+                           int64_t* batches, int64_t* screened, int64_t* dense_roots) {
+  return guard([&] {
+    if (dense_roots) *dense_roots = g_last_stats.dense;
     if (nroots) *nroots = g_last_stats.nroots;
     if (nnodes) *nnodes = g_last_stats.nnodes;
     if (nleaf_products) *nleaf_products = g_last_stats.nleaf_products;

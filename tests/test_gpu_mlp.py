@@ -1,0 +1,211 @@
+"""GPU checks of the learned path selector: the device network's forward pass,
+gradient and Adam steps against the CPU oracle (oracle/mlp.py, itself pinned
+by finite differences), node features against a brute-force restatement,
+learned plans executed with results identical to the oracle run on the same
+plan, the heuristic fallback without a model and the cost-model judge."""
+import numpy as np
+import pytest
+
+import oracle
+from oracle import mlp as omlp
+import strassen_amd as sa
+from strassen_amd import synth
+from gpu_util import ctx, pair_uniform, rel_fro
+
+pytestmark = pytest.mark.gpu
+F, O, NP = 10, 2, omlp.n_params(10, 2)
+
+
+def rand_model(seed):
+    rng = np.random.default_rng(seed)
+    prm = omlp.init(F, O, seed=seed) + 0.1 * rng.standard_normal(NP)
+    mu, sd = rng.random(F) * 0.1, 0.5 + rng.random(F)
+    return prm, mu, sd
+
+
+def test_info_matches_oracle_layout():
+    assert sa.mlp_info() == (F, O, NP)
+
+
+def test_forward_matches_oracle():
+    prm, mu, sd = rand_model(1)
+    sa.mlp_set(ctx(), prm, mu, sd)
+    X = np.random.default_rng(2).random((257, F))
+    y = sa.mlp_forward(ctx(), X)
+    ref = omlp.forward(prm, X, mu, sd, O)
+    assert np.max(np.abs(y - ref)) < 1e-13 * max(1.0, np.max(np.abs(ref)))
+
+
+def test_gradient_matches_oracle():
+    """Device backward kernels vs the oracle's analytic gradient (FP64; bar 1e-12
+    relative to the largest component), with a partial mask."""
+    prm, mu, sd = rand_model(3)
+    sa.mlp_set(ctx(), prm, mu, sd)
+    rng = np.random.default_rng(4)
+    X = rng.random((300, F)); T = rng.standard_normal((300, O)); M = (rng.random((300, O)) < 0.7).astype(float)
+    g, loss = sa.mlp_grad(ctx(), X, T, M)
+    gr = omlp.grad(prm, X, T, M, mu, sd)
+    assert np.max(np.abs(g - gr)) < 1e-12 * np.max(np.abs(gr))
+    assert abs(loss - omlp.loss(prm, X, T, M, mu, sd)) < 1e-13 * loss
+
+
+def test_adam_steps_match_oracle():
+    prm, mu, sd = rand_model(5)
+    sa.mlp_set(ctx(), prm, mu, sd)
+    rng = np.random.default_rng(6)
+    X = rng.random((128, F)); T = np.tanh(X @ rng.standard_normal((F, O)))
+    M = np.ones((128, O))
+    sa.mlp_train(ctx(), X, T, M, epochs=7, lr=5e-3)
+    p, m1, m2 = prm.copy(), np.zeros(NP), np.zeros(NP)
+    for t in range(1, 8):
+        p, m1, m2 = omlp.adam(p, omlp.grad(p, X, T, M, mu, sd), m1, m2, t, 5e-3)
+    got, _, _ = sa.mlp_get(ctx())
+    assert np.max(np.abs(got - p)) < 1e-11
+
+
+def test_training_reduces_loss():
+    prm, mu, sd = rand_model(7)
+    sa.mlp_set(ctx(), prm, mu, sd)
+    rng = np.random.default_rng(8)
+    X = rng.random((400, F)); T = np.stack([np.sin(3 * X[:, 0]) + X[:, 5], X[:, 3] - X[:, 4]], 1)
+    _, l0 = sa.mlp_grad(ctx(), X, T)
+    sa.mlp_train(ctx(), X, T, epochs=600, lr=3e-3)
+    _, l1 = sa.mlp_grad(ctx(), X, T)
+    assert l1 < 0.1 * l0
+
+
+def test_save_load_roundtrip(tmp_path):
+    prm, mu, sd = rand_model(9)
+    sa.mlp_set(ctx(), prm, mu, sd)
+    path = str(tmp_path / "m.txt")
+    sa.mlp_save(ctx(), path)
+    sa.mlp_set(ctx(), np.zeros(NP))
+    sa.mlp_load(ctx(), path)
+    p2, mu2, sd2 = sa.mlp_get(ctx())
+    assert np.array_equal(p2, prm) and np.array_equal(mu2, mu) and np.array_equal(sd2, sd)
+
+
+def problem(seed, nb, bs, fill=0.6, decay=0.3, cfill=1.0):
+    rng = np.random.default_rng(seed)
+    n = nb * bs
+    sz = synth.uniform_sizes(n, bs)
+    rb = np.repeat(np.arange(nb), sz)
+    a = rng.standard_normal((n, n)) * np.exp(-decay * np.abs(rb[:, None] - rb[None, :]))
+    b = rng.standard_normal((n, n))
+    a[~(rng.random((nb, nb)) < fill)[rb][:, rb]] = 0.0
+    b[~(rng.random((nb, nb)) < fill)[rb][:, rb]] = 0.0
+    c = np.where((rng.random((nb, nb)) < cfill)[rb][:, rb], 0.5, 0.0)
+    return n, np.asfortranarray(a), np.asfortranarray(b), np.asfortranarray(c)
+
+
+def test_node_features_brute_force():
+    nb, bs, thresh = 8, 16, 1e-6
+    n, a, b, c = problem(11, nb, bs, cfill=0.8)
+    oA, dA = pair_uniform(a, bs, bs, 0.0)
+    oB, dB = pair_uniform(b, bs, bs, 0.0)
+    oC, dC = pair_uniform(c, bs, bs, 0.0)
+    X = sa.strassen_features(dC, dA, dB, thresh)
+    pa, na = oA.meta(); pb, nbm = oB.meta(); pc, _ = oC.meta()
+    thr = thresh * bs ** 3
+    heur, _ = oracle.strassen_dryrun(oA, oB, thr)
+    rows, off = [], 0
+    for lev, sz in enumerate(oracle.tree_level_sizes(nb)):
+        w = 2 ** (lev + 1)
+        ns = -(-nb // w)
+        for s in range(sz):
+            si, sj, sk = s % ns, (s // ns) % ns, s // (ns * ns)
+            I, J, K = [range(x * w, min(x * w + w, nb)) for x in (si, sj, sk)]
+            raw = sum(2.0 * bs ** 3 for i in I for j in J for k in K
+                      if pa[i, k] and pb[k, j] and na[i, k] * nbm[k, j] >= thr)
+            Wd = w * bs
+            rows.append([lev / 8, np.log2(bs) / 8, np.log2(Wd) / 16,
+                         sum(pa[i, k] for i in I for k in K) / w ** 2, sum(pb[k, j] for k in K for j in J) / w ** 2,
+                         raw / (2.0 * Wd ** 3), sum(pc[i, j] for i in I for j in J) / w ** 2,
+                         float(heur[lev][s]), 0.0, np.log2(1 + raw) / 64])
+        off += sz
+    assert np.allclose(X, np.array(rows), rtol=1e-14, atol=1e-15)
+
+
+def run_plan_vs_oracle(seed, nb, bs, thresh=1e-10, **kw):
+    n, a, b, c = problem(seed, nb, bs, **kw)
+    oA, dA = pair_uniform(a, bs, bs, 0.0)
+    oB, dB = pair_uniform(b, bs, bs, 0.0)
+    oC, dC = pair_uniform(c, bs, bs, 0.0)
+    sbits, dbits, over, mc, hc = sa.strassen_plan(dC, dA, dB, thresh)
+    sa.matmult_strassen_sparse(dC, dA, dB, thresh, 0, opts=sa.default_opts(scheduler=sa.SCHED_MLP))
+    plan = sa.strassen_last_plan()
+    st = sa.strassen_last_stats()
+    if over:
+        leaf, nr = oracle.strassen_sparse(oC, oA, oB, thresh, 0)
+        nd = 0
+    else:
+        leaf, nr, nd = oracle.strassen_sparse(oC, oA, oB, thresh, 0, tree=sbits, dense=dbits, counts=True)
+    assert np.array_equal(leaf, ctx().tasklist())
+    assert (st["nroots"], st["dense_roots"]) == (nr, nd)
+    assert rel_fro(dC.to_dense(), oC.to_dense()) < 1e-12
+    return plan, over, nr, nd
+
+
+def test_learned_plan_executes_like_oracle():
+    """A model biased towards the dense and Strassen paths on full nodes: the
+    executed plan (read back with sa_strassen_plan) matches the oracle."""
+    prm = np.zeros(NP)
+    p = omlp.unpack(prm, F, O)
+    p["b3"][:] = [1.0, 1.0]           # both paths cost 2x the screened path ...
+    p["W3"][:, 0] = [-3.0, -4.0]      # ... minus a bonus growing with hidden unit 0
+    p["W1"][0, 3] = 4.0; p["W1"][0, 4] = 4.0; p["b1"][0] = -6.0   # unit 0 ~ tanh(4 fillA + 4 fillB - 6)
+    p["W2"][0, 0] = 2.0                                           # carried through the second layer
+    sa.mlp_set(ctx(), prm)
+    plan, over, nr, nd = run_plan_vs_oracle(21, 16, 16, fill=0.95)
+    assert not over and plan["scheduler"] == sa.SCHED_MLP
+    assert nr + nd > 0
+
+
+def test_fallback_without_model_and_judge():
+    fresh = sa.Context(0)
+    n, a, b, c = problem(31, 8, 16)
+    dA = sa.BSM.uniform(fresh, n, n, 16, 16, 0.0).from_dense(a)
+    dB = sa.BSM.uniform(fresh, n, n, 16, 16, 0.0).from_dense(b)
+    dC = sa.BSM.uniform(fresh, n, n, 16, 16, 0.0).from_dense(c)
+    sa.matmult_strassen_sparse(dC, dA, dB, 1e-10, 0, opts=sa.default_opts(scheduler=sa.SCHED_MLP))
+    assert sa.strassen_last_plan()["scheduler"] == sa.SCHED_HEURISTIC
+    # a model that prices Strassen and dense far below screened: on a sparse
+    # problem the reference cost model vetoes every such choice ...
+    prm = np.zeros(NP)
+    omlp.unpack(prm, F, O)["b3"][:] = [-20.0, -20.0]
+    sa.mlp_set(ctx(), prm)
+    plan, over, nr, nd = run_plan_vs_oracle(32, 16, 16, fill=0.2, decay=2.0)
+    assert not over and plan["scheduler"] == sa.SCHED_MLP and nr + nd == 0
+    # ... while on a full problem the same model's choices are admitted
+    plan, over, nr, nd = run_plan_vs_oracle(33, 16, 16, fill=1.0, decay=0.0)
+    assert not over and nr + nd > 0 and plan["model_cost"] <= 1.5 * 2.0 * (16 * 16) ** 3
+
+
+def test_trained_model_gain_over_heuristic():
+    """The committed model (trained on the method's own timings) against the
+    cost-model tree on a banded n=8192 problem of 128-blocks: the learned plan
+    must not be slower (median of 3, 10 % allowance for timing noise); the
+    measured gain is printed."""
+    import os
+    if not os.path.exists(sa.DEFAULT_PATH_MODEL):
+        pytest.skip("no trained path model in the tree")
+    c = sa.Context(0)
+    sa.mlp_load(c, sa.DEFAULT_PATH_MODEL)
+    w = synth.config5(n=8192, bs=128, band=24)
+    A = sa.BSM.uniform(c, w.n, w.n, 128, 128, w.thresh).generate(1, w.a_scale)
+    B = sa.BSM.uniform(c, w.n, w.n, 128, 128, w.thresh).generate(2, w.b_scale)
+    Cm = sa.BSM.uniform(c, w.n, w.n, 128, 128, 0.0)
+    Cm.fill(0.0)
+    times = {}
+    for name, sched in (("heuristic", sa.SCHED_HEURISTIC), ("learned", sa.SCHED_MLP)):
+        run = lambda: sa.matmult_strassen_sparse(Cm, A, B, w.thresh, 2048, opts=sa.default_opts(scheduler=sched))
+        run()
+        ts = []
+        for _ in range(3):
+            c.synchronize(); c.mark(0); run(); c.mark(1); c.synchronize()
+            ts.append(c.elapsed(0, 1))
+        times[name] = float(np.median(ts))
+    print(f"learned {times['learned']:.2f} ms vs heuristic {times['heuristic']:.2f} ms "
+          f"(gain {times['heuristic'] / times['learned']:.3f}x)")
+    assert times["learned"] <= 1.1 * times["heuristic"]
+

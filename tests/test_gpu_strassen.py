@@ -92,6 +92,51 @@ def test_random_inputs_vs_oracle(nb, bs, cutoff, kind, levels):
     assert rel_fro(cd, a @ b + 0.25) < 1e-12
 
 
+def random_plan(rng, nb, p_s, p_d):
+    sizes = oracle.tree_level_sizes(nb)
+    return ([(rng.random(s) < p_s).astype(np.uint8) for s in sizes],
+            [(rng.random(s) < p_d).astype(np.uint8) for s in sizes])
+
+
+def interior(nb, bits):
+    """Clear bits of nodes that reach past the block grid (edge super-blocks)."""
+    out = []
+    for lev, b in enumerate(bits):
+        w = 2 ** (lev + 1)
+        ns = -(-nb // w)
+        idx = np.arange(b.size)
+        si, sj, sk = idx % ns, (idx // ns) % ns, idx // (ns * ns)
+        ok = ((si + 1) * w <= nb) & ((sj + 1) * w <= nb) & ((sk + 1) * w <= nb)
+        out.append((b * ok).astype(np.uint8))
+    return out
+
+
+@pytest.mark.parametrize("seed,nb,bs,integer", [(1, 8, 16, True), (2, 16, 8, True), (3, 16, 32, False),
+                                                (4, 12, 16, False), (5, 32, 8, True)])
+def test_dense_roots_vs_oracle(seed, nb, bs, integer):
+    """Plans mixing Strassen and dense roots (random bits at every level):
+    screened leaf task lists bit-exact, integer inputs exact, random inputs
+    within the FP64 bar of the oracle run on the same plan."""
+    rng = np.random.default_rng(seed)
+    n, a, b = inputs(rng, nb, bs, "random", integer=integer)
+    tree, dense = random_plan(rng, nb, 0.1, 0.2)
+    tree, dense = interior(nb, tree), interior(nb, dense)
+    oA, dA = pair_uniform(a, bs, bs, 0.0)
+    oB, dB = pair_uniform(b, bs, bs, 0.0)
+    oC = oracle.OBsm.uniform(n, n, bs, bs, 0.0); oC.fill(0.5)
+    dC = sa.BSM.uniform(ctx(), n, n, bs, bs, 0.0); dC.fill(0.5)
+    leaf_o, nr, nd = oracle.strassen_sparse(oC, oA, oB, 1e-10, 0, tree=tree, dense=dense, counts=True)
+    sa.matmult_strassen_sparse(dC, dA, dB, 1e-10, 0, tree=tree, dense=dense)
+    st = sa.strassen_last_stats()
+    assert np.array_equal(leaf_o, ctx().tasklist())
+    assert (st["nroots"], st["dense_roots"]) == (nr, nd)
+    assert nd > 0
+    if integer:
+        assert np.array_equal(dC.to_dense(), oC.to_dense())
+    else:
+        assert rel_fro(dC.to_dense(), oC.to_dense()) < TOL
+
+
 def test_empty_c_blocks_drop_products():
     """C must be pre-filled; products into empty C blocks are dropped."""
     rng = np.random.default_rng(9)

@@ -278,6 +278,95 @@ def test_strassen_integer_exact(nb, bs, cutoff, kind, levels):
         assert nroots > 0
 
 
+def _brute_plan(a, b, sz, thr, roots, cmask):
+    """Product of integer-valued a, b by brute force over block triples: a triple
+    inside a Strassen or dense root region is always counted, any other one
+    only when it passes the screen; products into absent C blocks are dropped."""
+    nb = len(sz)
+    e = np.concatenate([[0], np.cumsum(sz)])
+    blk = lambda x, i, j: x[e[i]:e[i + 1], e[j]:e[j + 1]]
+    c = np.zeros_like(a)
+    for i in range(nb):
+        for j in range(nb):
+            if not cmask[i, j]:
+                continue
+            for k in range(nb):
+                ab, bb = blk(a, i, k), blk(b, k, j)
+                covered = any(r0 <= i < r0 + w and c0 <= j < c0 + w and k0 <= k < k0 + w
+                              for (r0, c0, k0, w) in roots)
+                na, nbm = np.linalg.norm(ab), np.linalg.norm(bb)
+                if covered or (na > 0 and nbm > 0 and na * nbm >= thr):
+                    c[e[i]:e[i + 1], e[j]:e[j + 1]] += ab @ bb
+    return c
+
+
+def _plan_roots(tree, dense, nb):
+    """Roots of a plan as the planner finds them: from the top level down, a
+    node with its Strassen bit (else its dense bit) set is a root covering its
+    region; otherwise its children are visited."""
+    roots = []
+    top = len(tree) - 1
+
+    def visit(lev, si, sj, sk):
+        w = 2 ** (lev + 1)
+        ns = -(-nb // w)
+        idx = si + sj * ns + sk * ns * ns
+        if tree[lev][idx] or dense[lev][idx]:
+            roots.append((si * w, sj * w, sk * w, w, "S" if tree[lev][idx] else "D"))
+            return
+        if lev == 0:
+            return
+        h = w // 2
+        for dj in range(2):
+            for di in range(2):
+                for dk in range(2):
+                    ci, cj, ck = 2 * si + di, 2 * sj + dj, 2 * sk + dk
+                    if ci * h < nb and cj * h < nb and ck * h < nb:
+                        visit(lev - 1, ci, cj, ck)
+
+    nt = -(-nb // 2 ** (top + 1))
+    for sk in range(nt):
+        for sj in range(nt):
+            for si in range(nt):
+                visit(top, si, sj, sk)
+    return roots
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3, 4, 5])
+def test_strassen_plan_dense_roots_integer_exact(seed):
+    """Random plans mixing Strassen and dense roots at every level (Strassen bit
+    wins where both are set) on integer inputs with absent A, B and C blocks and
+    a threshold that screens some leaf triples: bit-exact against a brute-force
+    sum over block triples."""
+    rng = np.random.default_rng(seed)
+    nb, bs = 8, 3
+    n = nb * bs
+    sz = synth.uniform_sizes(n, bs)
+    rbi = np.repeat(np.arange(nb), sz)
+    a = rng.integers(-3, 4, (n, n)).astype(float)
+    b = rng.integers(-3, 4, (n, n)).astype(float)
+    a[~(rng.random((nb, nb)) < 0.7)[rbi][:, rbi]] = 0.0
+    b[~(rng.random((nb, nb)) < 0.7)[rbi][:, rbi]] = 0.0
+    a *= np.exp2(-np.abs(rbi[:, None] - rbi[None, :]))  # decaying blocks so the screen binds; still exact
+    cmask = rng.random((nb, nb)) < 0.85
+    a = np.asfortranarray(a); b = np.asfortranarray(b)
+    A = oracle.OBsm.uniform(n, n, bs, bs, 0.0).from_dense(a)
+    B = oracle.OBsm.uniform(n, n, bs, bs, 0.0).from_dense(b)
+    c0 = np.asfortranarray(np.where(cmask[rbi][:, rbi], 1.0, 0.0))
+    Cm = oracle.OBsm.uniform(n, n, bs, bs, 0.0).from_dense(c0)
+    pres, _ = Cm.meta()
+    thresh = 2e-3
+    sizes = oracle.tree_level_sizes(nb)
+    tree = [(rng.random(s) < 0.15).astype(np.uint8) for s in sizes]
+    dense = [(rng.random(s) < 0.25).astype(np.uint8) for s in sizes]
+    _, nr, nd = oracle.strassen_sparse(Cm, A, B, thresh, 0, tree=tree, dense=dense, counts=True)
+    roots = _plan_roots(tree, dense, nb)
+    want = _brute_plan(a, b, sz, thresh * bs ** 3, [r[:4] for r in roots], pres.astype(bool)) + c0
+    assert np.array_equal(Cm.to_dense(), want)
+    assert nr == sum(r[4] == "S" for r in roots) and nd == sum(r[4] == "D" for r in roots)
+    assert nr + nd > 0
+
+
 @needs_ref
 @pytest.mark.parametrize("nb,bs,cutoff,kind", [(16, 4, 8, "dense"), (16, 4, 0, "band"), (12, 4, 0, "random")])
 def test_strassen_vs_reference(nb, bs, cutoff, kind):
@@ -355,3 +444,37 @@ def test_bsm_helpers_semantics():
     F = A.clone(); F.fill(0.5)
     _, nf = F.meta()
     assert np.allclose(nf, 1.5)
+
+
+# ----------------------------------------------------- path selector network --
+def test_mlp_oracle_gradient_matches_finite_differences():
+    """The numpy network's analytic gradient (the device kernels' check) is
+    pinned by central differences of the loss, masks included."""
+    from oracle import mlp
+    rng = np.random.default_rng(0)
+    F, O, n = 10, 2, 37
+    prm = mlp.init(F, O, seed=1) + 0.05 * rng.standard_normal(mlp.n_params(F, O))
+    X = rng.random((n, F)); T = rng.standard_normal((n, O)); M = (rng.random((n, O)) < 0.8).astype(float)
+    mu, sd = X.mean(0), X.std(0) + 0.1
+    g = mlp.grad(prm, X, T, M, mu, sd)
+    idx = rng.choice(prm.size, 60, replace=False)
+    fd = mlp.fd_grad(prm, X, T, M, mu, sd, idx)
+    assert np.max(np.abs(g[idx] - fd)) < 1e-8 * max(1.0, np.max(np.abs(fd)))
+    # a dropped mask entry removes that target from loss and gradient
+    M2 = M.copy(); M2[3, 1] = 0.0
+    T2 = T.copy(); T2[3, 1] += 100.0
+    assert np.allclose(mlp.grad(prm, X, T2, M2, mu, sd), mlp.grad(prm, X, T, M2, mu, sd))
+
+
+def test_mlp_oracle_adam_decreases_loss():
+    from oracle import mlp
+    rng = np.random.default_rng(2)
+    F, O, n = 10, 2, 64
+    X = rng.random((n, F)); W = rng.standard_normal((F, O)); T = np.tanh(X @ W)
+    M = np.ones((n, O)); mu, sd = X.mean(0), X.std(0) + 1e-3
+    prm = mlp.init(F, O, seed=3)
+    m1 = np.zeros_like(prm); m2 = np.zeros_like(prm)
+    l0 = mlp.loss(prm, X, T, M, mu, sd)
+    for t in range(1, 301):
+        prm, m1, m2 = mlp.adam(prm, mlp.grad(prm, X, T, M, mu, sd), m1, m2, t, 1e-2)
+    assert mlp.loss(prm, X, T, M, mu, sd) < 0.05 * l0
