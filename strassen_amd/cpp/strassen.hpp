@@ -52,6 +52,7 @@ void matmult_strassen_sparse(BlockSparseMatrix<Num>& C_mat, const BlockSparseMat
   sa_opts_t o;
   sa_opts_default(&o);
   o.cutoff = (int64_t)min_size_for_strassen;
+  o.scheduler = sa_cpp::learned() ? SA_SCHED_MLP : SA_SCHED_HEURISTIC;
   sa_cpp::check(sa_matmult_strassen_sparse(C_mat.handle(), A_mat.handle(), transA, B_mat.handle(), transB, thresh,
                                            alpha, beta, &o),
                 "sa_matmult_strassen_sparse");
