@@ -39,6 +39,7 @@ from strassen_amd import synth  # noqa: E402
 
 PEAK_FP64 = 78.6     # TFLOP/s, MI355X dense FP64 (matrix)
 PEAK_FP16 = 2500.0   # TFLOP/s, MI355X dense FP16/BF16 matrix
+PEAK_INT8 = 5000.0   # TOP/s, MI355X dense int8 matrix
 METRIC = "dense-equiv TFLOP/s & RI-K step time: mixed vs FP64 GPU at <=1e-10 rel err"
 
 
@@ -110,8 +111,9 @@ def bench_rik(args):
     dBp = sa.BSM(ctx, w.p_sizes, bcs, 0.0)
     dK = sa.BSM(ctx, w.mu_sizes, w.mu_sizes, 0.0)
     o64 = sa.default_opts()
-    oz_t = sa.default_opts(mode=sa.MODE_OZAKI, slices=args.slices_transform)
-    oz_k = sa.default_opts(mode=sa.MODE_OZAKI, slices=args.slices_k)
+    st = {"fp16": sa.SLICE_FP16, "bf16": sa.SLICE_BF16, "int8": sa.SLICE_INT8}[args.slice_type]
+    oz_t = sa.default_opts(mode=sa.MODE_OZAKI, slices=args.slices_transform, slice_type=st)
+    oz_k = sa.default_opts(mode=sa.MODE_OZAKI, slices=args.slices_k, slice_type=st)
 
     # algorithmic flops from the task lists (transform, then K)
     log(f"B generated in HBM ({dB.info()[5] / 1e9:.1f} GB)")
@@ -164,6 +166,7 @@ def bench_rik(args):
     mfma_fl = pairs_t * f_tr + pairs_k * f_k       # low-precision MFMA work per step
     oz_per_step_ms = goz_ms / args.steps
     achieved_lp = mfma_fl / (oz_per_step_ms * 1e-3) / 1e12
+    peak_lp = PEAK_INT8 if args.slice_type == "int8" else PEAK_FP16
     fp64_achieved = (f_tr + f_k) / (g64_ms / args.steps * 1e-3) / 1e12
 
     rec = {
@@ -173,13 +176,13 @@ def bench_rik(args):
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(msmix, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64 with fp16-slice products",
+        "dtype": f"f64 with {args.slice_type}-slice products",
         "data": "synthetic (seeded geometry; B generated in HBM from a counter-based hash)",
         "config": {"workload": w.name, "natoms": w.natoms, "n_bf": w.n_bf, "n_aux": w.n_aux, "n_occ": w.n_occ,
                    "p_groups": int(len(w.p_sizes)), "mu_groups": int(len(w.mu_sizes)),
                    "i_chunks": [int(x) for x in w.i_chunks], "b_block_fill": round(float((w.b_scale > 0).mean()), 4),
                    "thresh": w.thresh, "slices_transform": args.slices_transform, "slices_k": args.slices_k,
-                   "slice_type": "fp16", "parallelism": "single GPU"},
+                   "slice_type": args.slice_type, "parallelism": "single GPU"},
         "ri_k": {"step_ms_split_precision": round(msmix, 3), "step_ms_fp64": round(ms64, 3),
                  "speedup_vs_fp64_gpu": round(ms64 / msmix, 3), "rel_err_K_vs_fp64": err_k,
                  "dense_equiv_tflops_fp64": round(dense_fl / (ms64 * 1e-3) / 1e12, 3),
@@ -193,8 +196,9 @@ def bench_rik(args):
                                        "norms": round(nrm_ms / args.steps, 3)},
                  "setup": {"metric_host_s": round(t_metric, 3), "inv_sqrt_s": round(t_ns, 3),
                            "newton_schulz_iterations": ns_it, "newton_schulz_residual": ns_res}},
-        "roofline": {"bound": "mfma", "achieved": round(achieved_lp, 3), "peak": PEAK_FP16, "unit": "TFLOP/s",
-                     "frac": round(achieved_lp / PEAK_FP16, 4), "traffic": None, "kernel": "k_gemm_ozaki",
+        "roofline": {"bound": "mfma", "achieved": round(achieved_lp, 3), "peak": peak_lp,
+                     "unit": "TOP/s" if args.slice_type == "int8" else "TFLOP/s",
+                     "frac": round(achieved_lp / peak_lp, 4), "traffic": None, "kernel": "k_gemm_ozaki",
                      "kernel_avg_ms": round(goz_ms / max(goz_n, 1), 3),
                      "note": "achieved = executed FP16 slice-product flops (pairs x algorithmic) / kernel time; "
                              "FP64 path k_gemm_fp64 reported in ri_k.fp64_gemm_tflops (peak 78.6)"},
@@ -368,6 +372,7 @@ def main():
     ap.add_argument("--tile", type=int, default=-1)
     ap.add_argument("--slices-transform", type=int, default=4)
     ap.add_argument("--slices-k", type=int, default=4)
+    ap.add_argument("--slice-type", default="fp16", choices=["fp16", "bf16", "int8"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--n-occ", type=int, default=200, help="RI-K occupied orbitals (200 = the headline workload)")
     ap.add_argument("--no-check", action="store_true", help="config5: skip the learned-vs-heuristic comparison")

@@ -40,14 +40,15 @@ typedef struct sa_bsm* sa_bsm_t;
  * semantics in FP64). */
 #define SA_MODE_FP64 0
 #define SA_MODE_OZAKI 1
-#define SA_SLICE_FP16 0
-#define SA_SLICE_BF16 1
+#define SA_SLICE_FP16 0 /* 10-bit digits in FP16, FP32 MFMA accumulation (3..5 slices) */
+#define SA_SLICE_BF16 1 /* 8-bit digits in BF16, FP32 accumulation (4..6 slices) */
+#define SA_SLICE_INT8 2 /* 8-bit digits in int8, exact int32 accumulation (4..6 slices) */
 #define SA_SCHED_HEURISTIC 0 /* dry-run cost model, strassen.hpp:579-673 */
 #define SA_SCHED_MLP 1       /* learned per-node selector, heuristic as fallback */
 #define SA_SCHED_TREE 2      /* caller-supplied decision tree bits */
 typedef struct {
   int32_t mode;        /* SA_MODE_* */
-  int32_t slices;      /* Ozaki: slices per operand (FP16: 3..5, BF16: 4..6) */
+  int32_t slices;      /* Ozaki: slices per operand (FP16: 3..5, BF16 and INT8: 4..6) */
   int32_t slice_type;  /* SA_SLICE_* */
   int32_t tile;        /* FP64: GEMM tile config id (sa_ctx_gemm_config), -1 = autotuned;
                           Ozaki: kernel table id, ignored unless it runs this slice count/type */

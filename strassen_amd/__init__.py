@@ -20,7 +20,7 @@ LIB_PATH = os.path.join(PKG, "lib", "libstrassen_amd.so")
 
 SA_OK = 0
 MODE_FP64, MODE_OZAKI = 0, 1
-SLICE_FP16, SLICE_BF16 = 0, 1
+SLICE_FP16, SLICE_BF16, SLICE_INT8 = 0, 1, 2
 SCHED_HEURISTIC, SCHED_MLP, SCHED_TREE = 0, 1, 2
 
 _lib = None

@@ -20,14 +20,19 @@
 //    weight 1 for 2 (2*256*512*32*2 = 2^24); BF16 weights 0..2 for 8.  Later
 //    weights are folded on the same schedule and may round in FP32, which
 //    costs at most 2^-(2 beta + 21) of the leading term per fold.
+// INT8 slices (an MI355X-first third form, v_mfma_i32_16x16x64_i8 at twice
+// the FP16 rate): beta = 8, TS = 2 digits in int8; digit products accumulate
+// in int32, exact for 2^11 / S k-steps of 64 (|d_p d_q| <= 2^14), so products
+// are folded only at their end (or every 256 k-steps).
 // Truncation error per product is about 2^-(beta*S - TS) relative to the
-// row/column maxima of the two blocks (4 FP16 slices: 2^-38).
+// row/column maxima of the two blocks (4 FP16 slices: 2^-38; 5 INT8: 2^-38).
 //
-// Slice layout: for each block, S slices of [vb][ks][lane][8] fragments, one
-// fragment = 16 vectors x 32 k = the operand of one 16x16x32 MFMA (1 KB):
-// lane l holds vector 16 vb + (l & 15) at k = 32 ks + 8 (l >> 4) + t.  Vectors
-// are padded to a multiple of 16 and k to a multiple of 32 with zeros, so every
-// fragment load is one contiguous, coalesced 1 KB read.
+// Slice layout: for each block, S slices of [vb][ks][lane][16 bytes]
+// fragments, one fragment = 16 vectors x KS k = the operand of one 16x16xKS
+// MFMA (1 KB; KS = 32 for FP16/BF16, 64 for INT8): lane l holds vector
+// 16 vb + (l & 15) at k = KS ks + EL (l >> 4) + t, EL = 8 (16-bit digits) or
+// 16 (8-bit).  Vectors are padded to a multiple of 16 and k to a multiple of
+// KS with zeros, so every fragment load is one contiguous, coalesced 1 KB read.
 #include <algorithm>
 #include <cmath>
 
@@ -40,9 +45,36 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float flt4 __attribute__((ext_vector_type(4)));
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
-constexpr int kOzK = 32;  // MFMA k of the 16x16x32 low-precision instructions
-__host__ __device__ constexpr int oz_beta(bool bf) { return bf ? 8 : 10; }
-__host__ __device__ constexpr int oz_tshift(bool bf) { return bf ? 1 : 2; }
+typedef int int4v __attribute__((ext_vector_type(4)));
+
+// slice kinds: SA_SLICE_FP16 (0), SA_SLICE_BF16 (1), SA_SLICE_INT8 (2)
+__host__ __device__ constexpr int oz_beta(int kind) { return kind == 0 ? 10 : 8; }
+__host__ __device__ constexpr int oz_tshift(int kind) { return kind == 1 ? 1 : 2; }
+__host__ __device__ constexpr int oz_kstep(int kind) { return kind == 2 ? 64 : 32; }  // MFMA k
+__host__ __device__ constexpr int oz_esize(int kind) { return kind == 2 ? 1 : 2; }    // bytes per digit
+// k-steps between folds of all weights (FP16 also folds weights 1..2 every 2)
+__host__ __device__ constexpr int oz_full_fold(int kind) { return kind == 2 ? 256 : 8; }
+
+template <int KIND>
+struct OzAcc {
+  typedef flt4 type;
+};
+template <>
+struct OzAcc<2> {
+  typedef int4v type;
+};
+
+__device__ inline flt4 oz_mfma(std::integral_constant<int, 0>, uint4 a, uint4 b, flt4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(half8, a), __builtin_bit_cast(half8, b), c, 0, 0,
+                                                0);
+}
+__device__ inline flt4 oz_mfma(std::integral_constant<int, 1>, uint4 a, uint4 b, flt4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0,
+                                                 0, 0);
+}
+__device__ inline int4v oz_mfma(std::integral_constant<int, 2>, uint4 a, uint4 b, int4v c) {
+  return __builtin_amdgcn_mfma_i32_16x16x64_i8(__builtin_bit_cast(int4v, a), __builtin_bit_cast(int4v, b), c, 0, 0, 0);
+}
 __host__ __device__ constexpr double pow2i(int e) {
   double r = 1.0;
   for (int i = 0; i < e; ++i) r *= 2.0;
@@ -51,11 +83,11 @@ __host__ __device__ constexpr double pow2i(int e) {
 }
 
 struct OzProd {
-  const uint16_t* a;  // S slices of the op(A) block (fragment layout)
-  const uint16_t* b;  // S slices of the op(B) block
+  const uint8_t* a;   // S slices of the op(A) block (fragment layout)
+  const uint8_t* b;   // S slices of the op(B) block
   const int32_t* ea;  // row exponents (padded to 16)
   const int32_t* eb;  // column exponents (padded to 16)
-  int32_t nks;        // k-steps of 32
+  int32_t nks;        // MFMA k-steps
   int32_t nvba;       // 16-row blocks of the op(A) block
   int32_t nvbb;       // 16-column blocks of the op(B) block
   int32_t pad;
@@ -85,35 +117,38 @@ __device__ inline void slice_geom(const SliceArgs& s, int64_t b, int& nv, int& k
   }
 }
 
-__global__ void k_slice_need(SliceArgs s, int S, int64_t* __restrict__ need_sl, int64_t* __restrict__ need_ex) {
+// bytes of slices and number of exponents of every block
+__global__ void k_slice_need(SliceArgs s, int S, int kstep, int esize, int64_t* __restrict__ need_sl,
+                             int64_t* __restrict__ need_ex) {
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= s.nb) return;
   int nv, kd;
   int64_t sv, sk;
   slice_geom(s, b, nv, kd, sv, sk);
   const bool pres = s.off[b] >= 0;
-  const int64_t kp = ceil_div(kd, kOzK) * kOzK;
+  const int64_t kp = ceil_div(kd, kstep) * kstep;
   const int64_t nvp = ceil_div(nv, 16) * 16;
-  need_sl[b] = pres ? (int64_t)S * nvp * kp : 0;
+  need_sl[b] = pres ? (int64_t)S * nvp * kp * esize : 0;
   need_ex[b] = pres ? nvp : 0;
 }
 
-template <bool BF>
+template <int KIND>
 __global__ void __launch_bounds__(256) k_slice(SliceArgs s, int S, const int64_t* __restrict__ sl_off,
-                                               const int64_t* __restrict__ ex_off, uint16_t* __restrict__ slices,
+                                               const int64_t* __restrict__ ex_off, uint8_t* __restrict__ slices,
                                                int32_t* __restrict__ exps) {
-  constexpr int BETA = oz_beta(BF), TS = oz_tshift(BF);
+  constexpr int BETA = oz_beta(KIND), TS = oz_tshift(KIND);
+  constexpr int KS = oz_kstep(KIND), ES = oz_esize(KIND), EL = 16 / ES;
   const int64_t b = blockIdx.x;
   const int64_t o = s.off[b];
   if (o < 0) return;
   int nv, kd;
   int64_t sv, sk;
   slice_geom(s, b, nv, kd, sv, sk);
-  const int nks = (int)ceil_div(kd, kOzK), kp = nks * kOzK;
+  const int nks = (int)ceil_div(kd, KS), kp = nks * KS;
   const int nvb = (int)ceil_div(nv, 16), nvp = nvb * 16;
-  const int64_t pstride = (int64_t)nvp * kp;
+  const int64_t pstride = (int64_t)nvp * kp * ES;  // bytes per slice
   const double* x = s.arena + o;
-  uint16_t* out = slices + sl_off[b];
+  uint8_t* out = slices + sl_off[b];
   int32_t* ex = exps + ex_off[b];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int64_t mask = ((int64_t)1 << BETA) - 1, half = (int64_t)1 << (BETA - 1);
@@ -126,11 +161,11 @@ __global__ void __launch_bounds__(256) k_slice(SliceArgs s, int S, const int64_t
     int e = 0;
     if (mx > 0.0) (void)frexp(mx, &e);  // mx < 2^e
     if (lane == 0) ex[v] = e;
-    const int64_t vbase = (int64_t)(v >> 4) * nks * 512 + (v & 15) * 8;
+    const int64_t vbase = (int64_t)(v >> 4) * nks * 1024 + (v & 15) * 16;
     for (int kk = lane; kk < kp; kk += 64) {
       const double a = (v < nv && kk < kd) ? x[v * sv + kk * sk] : 0.0;
       int64_t n = mx > 0.0 ? (int64_t)rint(ldexp(a, BETA * S - TS - e)) : 0;
-      const int64_t at = vbase + (int64_t)(kk >> 5) * 512 + ((kk & 31) >> 3) * 128 + (kk & 7);
+      const int64_t at = vbase + (int64_t)(kk / KS) * 1024 + ((kk % KS) / EL) * 256 + (kk % EL) * ES;
       // balanced digits, least significant first
       for (int p = S - 1; p >= 0; --p) {
         int64_t d;
@@ -141,15 +176,14 @@ __global__ void __launch_bounds__(256) k_slice(SliceArgs s, int S, const int64_t
         } else {
           d = n;
         }
-        uint16_t bits;
-        if (BF) {
-          const __bf16 h = (__bf16)(float)d;
-          bits = __builtin_bit_cast(uint16_t, h);
+        uint8_t* dst = out + (int64_t)p * pstride + at;
+        if (KIND == 2) {
+          *dst = (uint8_t)(int8_t)d;
         } else {
-          const _Float16 h = (_Float16)(float)d;
-          bits = __builtin_bit_cast(uint16_t, h);
+          const uint16_t bits = KIND == 1 ? __builtin_bit_cast(uint16_t, (__bf16)(float)d)
+                                          : __builtin_bit_cast(uint16_t, (_Float16)(float)d);
+          *reinterpret_cast<uint16_t*>(dst) = bits;
         }
-        out[(int64_t)p * pstride + at] = bits;
       }
     }
   }
@@ -175,14 +209,15 @@ struct OzTables {
   int32_t b_nrb;
   const int32_t* ksz;
   int32_t ta, tb;
-  const uint16_t* asl;
+  const uint8_t* asl;
   const int64_t* asl_off;
   const int32_t* aex;
   const int64_t* aex_off;
-  const uint16_t* bsl;
+  const uint8_t* bsl;
   const int64_t* bsl_off;
   const int32_t* bex;
   const int64_t* bex_off;
+  int32_t kstep;
 };
 
 __global__ void k_oz_records(OzTables t, GemmOut* __restrict__ outs, OzProd* __restrict__ prods) {
@@ -218,7 +253,7 @@ __global__ void k_oz_records(OzTables t, GemmOut* __restrict__ outs, OzProd* __r
     pr.b = t.bsl + t.bsl_off[bi];
     pr.ea = t.aex + t.aex_off[ai];
     pr.eb = t.bex + t.bex_off[bi];
-    pr.nks = (int32_t)ceil_div(t.ksz[kb], kOzK);
+    pr.nks = (int32_t)ceil_div(t.ksz[kb], t.kstep);
     pr.nvba = (int32_t)ceil_div(o.m, 16);
     pr.nvbb = (int32_t)ceil_div(o.n, 16);
     pr.pad = 0;
@@ -236,15 +271,17 @@ __global__ void k_oz_records(OzTables t, GemmOut* __restrict__ outs, OzProd* __r
 // the output block hold zeros (their loads are skipped) and run through the
 // MFMAs like the others: a branch per fragment would put every MFMA in its own
 // basic block and serialise it behind the LDS reads.
-template <int S, bool BF, int FM, int FN>
+template <int S, int KIND, int FM, int FN>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
     k_gemm_ozaki(const GemmOut* __restrict__ outs, const OzProd* __restrict__ prods,
                  const GemmItem* __restrict__ items, int64_t nitems, double alpha) {
-  constexpr int BETA = oz_beta(BF), TS = oz_tshift(BF);
+  constexpr int BETA = oz_beta(KIND), TS = oz_tshift(KIND);
+  typedef typename OzAcc<KIND>::type acc4;
   constexpr int RA = 2 * FM, RB = 2 * FN;    // 16-blocks of rows / columns per tile
   constexpr int NCA = S * RA, NCB = S * RB;  // fragments per stage
   constexpr int LA = (NCA + 3) / 4, LB = (NCB + 3) / 4;
-  constexpr int FREQ_LO = 1, FREQ_HI = (BF || S < 3) ? 1 : 3;  // weights folded every 2 k-steps (FP16)
+  constexpr int FREQ_LO = 1, FREQ_HI = (KIND != 0 || S < 3) ? 1 : 3;  // weights folded every 2 k-steps (FP16)
+  constexpr int FULL = oz_full_fold(KIND);
   __shared__ uint4 As[2][NCA][64];
   __shared__ uint4 Bs[2][NCB][64];
 
@@ -265,14 +302,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
   for (int j = 0; j < FN; ++j) act_j[j] = cb0 + wn * FN + j < ncb;
 
   dbl4 acc[FM][FN];
-  flt4 s32[S][FM][FN];
+  acc4 s32[S][FM][FN];
 #pragma unroll
   for (int i = 0; i < FM; ++i)
 #pragma unroll
     for (int j = 0; j < FN; ++j) {
       acc[i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-      for (int w = 0; w < S; ++w) s32[w][i][j] = flt4{0.f, 0.f, 0.f, 0.f};
+      for (int w = 0; w < S; ++w) s32[w][i][j] = acc4{0, 0, 0, 0};
     }
   int er[FM][4], ec[FN];
   uint4 ra[LA], rbf[LB];
@@ -333,12 +370,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
         for (int i = 0; i < FM; ++i)
 #pragma unroll
           for (int j = 0; j < FN; ++j) {
-            if (BF)
-              s32[pa + pb][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                  __builtin_bit_cast(bf16x8, aq[i]), __builtin_bit_cast(bf16x8, bq[pb][j]), s32[pa + pb][i][j], 0, 0, 0);
-            else
-              s32[pa + pb][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
-                  __builtin_bit_cast(half8, aq[i]), __builtin_bit_cast(half8, bq[pb][j]), s32[pa + pb][i][j], 0, 0, 0);
+            s32[pa + pb][i][j] = oz_mfma(std::integral_constant<int, KIND>{}, aq[i], bq[pb][j], s32[pa + pb][i][j]);
           }
     }
   };
@@ -356,7 +388,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 #pragma unroll
           for (int v = 0; v < 4; ++v)
             acc[i][j][v] += ldexp((double)s32[w][i][j][v], er[i][v] + ec[j] + 2 * TS - BETA * (w + 2));
-          s32[w][i][j] = flt4{0.f, 0.f, 0.f, 0.f};
+          s32[w][i][j] = acc4{0, 0, 0, 0};
         }
     }
   };
@@ -390,7 +422,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
       compute(buf);
       const int done = ks + 1;
       int lo = S, hi = 0;
-      if (np != p || (done & 7) == 0) {
+      if (np != p || (done & (FULL - 1)) == 0) {
         lo = 0;
         hi = S;
       } else if (FREQ_HI > FREQ_LO && (done & 1) == 0) {
@@ -441,17 +473,18 @@ struct OzCursor {
   int32_t p, ks, nks;
 };
 
-template <int S, bool BF, int WM, int WN, int FM, int FN>
+template <int S, int KIND, int WM, int WN, int FM, int FN>
 __global__ void __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_eu(2, 2)))
     k_gemm_ozaki_deep(const GemmOut* __restrict__ outs, const OzProd* __restrict__ prods,
                       const GemmItem* __restrict__ items, int64_t nitems, double alpha) {
-  constexpr int BETA = oz_beta(BF), TS = oz_tshift(BF);
+  constexpr int BETA = oz_beta(KIND), TS = oz_tshift(KIND);
+  typedef typename OzAcc<KIND>::type acc4;
   constexpr int NW = WM * WN;
   constexpr int RA = WM * FM, RB = WN * FN;
   constexpr int NCA = S * RA, NCB = S * RB;
   constexpr int LA = (NCA + NW - 1) / NW, LB = (NCB + NW - 1) / NW;
-  // FP16: weight 1 is exact for 2 k-steps of 32, BF16 weights 0..2 for 8
-  constexpr int FOLD_EVERY = BF ? 8 : 2;
+  // FP16: weight 1 is exact for 2 k-steps of 32, BF16 weights 0..2 for 8, INT8 all for 256 of 64
+  constexpr int FOLD_EVERY = KIND == 0 ? 2 : oz_full_fold(KIND);
   __shared__ uint4 As[2][NCA][64];
   __shared__ uint4 Bs[2][NCB][64];
 
@@ -472,14 +505,14 @@ __global__ void __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_
   for (int j = 0; j < FN; ++j) act_j[j] = cb0 + wn * FN + j < ncb;
 
   dbl4 acc[FM][FN];
-  flt4 s32[S][FM][FN];
+  acc4 s32[S][FM][FN];
 #pragma unroll
   for (int i = 0; i < FM; ++i)
 #pragma unroll
     for (int j = 0; j < FN; ++j) {
       acc[i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-      for (int w = 0; w < S; ++w) s32[w][i][j] = flt4{0.f, 0.f, 0.f, 0.f};
+      for (int w = 0; w < S; ++w) s32[w][i][j] = acc4{0, 0, 0, 0};
     }
   int er[FM][4], ec[FN];
   uint4 ra[2][LA], rbf[2][LB];
@@ -545,12 +578,7 @@ __global__ void __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_
         for (int i = 0; i < FM; ++i)
 #pragma unroll
           for (int j = 0; j < FN; ++j) {
-            if (BF)
-              s32[pa + pb][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                  __builtin_bit_cast(bf16x8, aq[i]), __builtin_bit_cast(bf16x8, bq[pb][j]), s32[pa + pb][i][j], 0, 0, 0);
-            else
-              s32[pa + pb][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
-                  __builtin_bit_cast(half8, aq[i]), __builtin_bit_cast(half8, bq[pb][j]), s32[pa + pb][i][j], 0, 0, 0);
+            s32[pa + pb][i][j] = oz_mfma(std::integral_constant<int, KIND>{}, aq[i], bq[pb][j], s32[pa + pb][i][j]);
           }
     }
   };
@@ -570,7 +598,7 @@ __global__ void __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_
           acc[i][j][v] += ldexp(t, er[i][v] + ec[j] + 2 * TS - 2 * BETA);
         }
 #pragma unroll
-        for (int w = 0; w < S; ++w) s32[w][i][j] = flt4{0.f, 0.f, 0.f, 0.f};
+        for (int w = 0; w < S; ++w) s32[w][i][j] = acc4{0, 0, 0, 0};
       }
   };
 
@@ -651,26 +679,30 @@ using OzFn = void (*)(const GemmOut*, const OzProd*, const GemmItem*, int64_t, d
 
 struct OzCfg {
   int S;
-  bool bf;
+  int kind;
   int fm, fn;
   bool deep;
   int threads;
   OzFn fn_;
 };
 
-#define SA_OZ(S, BF, FM, FN) OzCfg{S, BF, FM, FN, false, 256, k_gemm_ozaki<S, BF, FM, FN>}
+#define SA_OZ(S, K, FM, FN) OzCfg{S, K, FM, FN, false, 256, k_gemm_ozaki<S, K, FM, FN>}
 // deep: (WM x WN) = (RM x RN) waves of 32 x 32, workgroup tile (32 RM) x (32 RN)
-#define SA_OZD(S, BF, RM, RN) OzCfg{S, BF, RM, RN, true, 64 * RM * RN, k_gemm_ozaki_deep<S, BF, RM, RN, 2, 2>}
+#define SA_OZD(S, K, RM, RN) OzCfg{S, K, RM, RN, true, 64 * RM * RN, k_gemm_ozaki_deep<S, K, RM, RN, 2, 2>}
 
 static const OzCfg kOzCfgs[] = {
-    SA_OZ(3, false, 2, 2),  SA_OZ(4, false, 2, 2),  SA_OZ(5, false, 2, 2),   // 0-2: 64x64
-    SA_OZ(3, false, 1, 1),  SA_OZ(4, false, 1, 1),  SA_OZ(5, false, 1, 1),   // 3-5: 32x32
-    SA_OZ(4, true, 2, 2),   SA_OZ(5, true, 2, 2),   SA_OZ(6, true, 2, 2),    // 6-8
-    SA_OZ(4, true, 1, 1),   SA_OZ(5, true, 1, 1),   SA_OZ(6, true, 1, 1),    // 9-11
-    SA_OZD(3, false, 4, 2), SA_OZD(4, false, 4, 2), SA_OZD(5, false, 4, 2),  // 12-14: 128x64, deep
-    SA_OZD(3, false, 2, 4), SA_OZD(4, false, 2, 4), SA_OZD(5, false, 2, 4),  // 15-17: 64x128, deep
-    SA_OZD(4, true, 4, 2),  SA_OZD(5, true, 4, 2),                           // 18-19
-    SA_OZD(4, true, 2, 4),  SA_OZD(5, true, 2, 4),                           // 20-21
+    SA_OZ(3, 0, 2, 2),  SA_OZ(4, 0, 2, 2),  SA_OZ(5, 0, 2, 2),   // 0-2: FP16 64x64
+    SA_OZ(3, 0, 1, 1),  SA_OZ(4, 0, 1, 1),  SA_OZ(5, 0, 1, 1),   // 3-5: FP16 32x32
+    SA_OZ(4, 1, 2, 2),  SA_OZ(5, 1, 2, 2),  SA_OZ(6, 1, 2, 2),   // 6-8: BF16 64x64
+    SA_OZ(4, 1, 1, 1),  SA_OZ(5, 1, 1, 1),  SA_OZ(6, 1, 1, 1),   // 9-11: BF16 32x32
+    SA_OZD(3, 0, 4, 2), SA_OZD(4, 0, 4, 2), SA_OZD(5, 0, 4, 2),  // 12-14: FP16 128x64, deep
+    SA_OZD(3, 0, 2, 4), SA_OZD(4, 0, 2, 4), SA_OZD(5, 0, 2, 4),  // 15-17: FP16 64x128, deep
+    SA_OZD(4, 1, 4, 2), SA_OZD(5, 1, 4, 2),                      // 18-19: BF16 128x64, deep
+    SA_OZD(4, 1, 2, 4), SA_OZD(5, 1, 2, 4),                      // 20-21: BF16 64x128, deep
+    SA_OZ(4, 2, 2, 2),  SA_OZ(5, 2, 2, 2),  SA_OZ(6, 2, 2, 2),   // 22-24: INT8 64x64
+    SA_OZ(4, 2, 1, 1),  SA_OZ(5, 2, 1, 1),  SA_OZ(6, 2, 1, 1),   // 25-27: INT8 32x32
+    SA_OZD(4, 2, 4, 2), SA_OZD(5, 2, 4, 2),                      // 28-29: INT8 128x64, deep
+    SA_OZD(4, 2, 2, 4), SA_OZD(5, 2, 2, 4),                      // 30-31: INT8 64x128, deep
 };
 
 __global__ void k_count_tiles_oz(const GemmOut* __restrict__ outs, int64_t nouts, int TM, int TN,
@@ -692,13 +724,13 @@ __global__ void k_emit_items_oz(const GemmOut* __restrict__ outs, int64_t nouts,
 
 // slice one operand (all present blocks); returns device pointers
 struct Sliced {
-  const uint16_t* sl;
+  const uint8_t* sl;
   const int64_t* sl_off;
   const int32_t* ex;
   const int64_t* ex_off;
 };
 
-static Sliced slice_operand(sa_context* c, sa_bsm* X, bool role_b, bool trans, int S, bool bf, const char* tag) {
+static Sliced slice_operand(sa_context* c, sa_bsm* X, bool role_b, bool trans, int S, int kind, const char* tag) {
   const int64_t nb = X->nb();
   std::string t(tag);
   DevBuf& bso = c->buf((t + "_sloff").c_str());
@@ -708,8 +740,8 @@ static Sliced slice_operand(sa_context* c, sa_bsm* X, bool role_b, bool trans, i
   beo.reserve(sizeof(int64_t) * (size_t)(nb + 1));
   btot.reserve(sizeof(int64_t) * 2);
   SliceArgs a{X->d_off, X->data(), X->d_rsz, X->d_csz, X->nrb, nb, role_b ? 1 : 0, trans ? 1 : 0};
-  hipLaunchKernelGGL(k_slice_need, dim3((unsigned)ceil_div(nb, 256)), dim3(256), 0, c->stream, a, S,
-                     bso.as<int64_t>(), beo.as<int64_t>());
+  hipLaunchKernelGGL(k_slice_need, dim3((unsigned)ceil_div(nb, 256)), dim3(256), 0, c->stream, a, S, oz_kstep(kind),
+                     oz_esize(kind), bso.as<int64_t>(), beo.as<int64_t>());
   scan_exclusive_i64_async(c, bso.as<int64_t>(), nb, btot.as<int64_t>());
   scan_exclusive_i64_async(c, beo.as<int64_t>(), nb, btot.as<int64_t>() + 1);
   int64_t* h = static_cast<int64_t*>(pinned(c, 2 * sizeof(int64_t)));
@@ -718,21 +750,23 @@ static Sliced slice_operand(sa_context* c, sa_bsm* X, bool role_b, bool trans, i
   const int64_t nsl = h[0], nex = h[1];
   DevBuf& bsl = c->buf((t + "_sl").c_str());
   DevBuf& bex = c->buf((t + "_ex").c_str());
-  bsl.reserve(sizeof(uint16_t) * (size_t)(nsl + 8));
+  bsl.reserve((size_t)(nsl + 64));
   bex.reserve(sizeof(int32_t) * (size_t)(nex + 1));
   {
     ProfScope ps(c, "ozaki_slice");
     if (nb) {
-      if (bf)
-        hipLaunchKernelGGL(k_slice<true>, dim3((unsigned)nb), dim3(256), 0, c->stream, a, S,
-                           (const int64_t*)bso.p, (const int64_t*)beo.p, bsl.as<uint16_t>(), bex.as<int32_t>());
+      const dim3 g((unsigned)nb), t(256);
+      const int64_t *so = (const int64_t*)bso.p, *eo = (const int64_t*)beo.p;
+      if (kind == 2)
+        hipLaunchKernelGGL(k_slice<2>, g, t, 0, c->stream, a, S, so, eo, bsl.as<uint8_t>(), bex.as<int32_t>());
+      else if (kind == 1)
+        hipLaunchKernelGGL(k_slice<1>, g, t, 0, c->stream, a, S, so, eo, bsl.as<uint8_t>(), bex.as<int32_t>());
       else
-        hipLaunchKernelGGL(k_slice<false>, dim3((unsigned)nb), dim3(256), 0, c->stream, a, S,
-                           (const int64_t*)bso.p, (const int64_t*)beo.p, bsl.as<uint16_t>(), bex.as<int32_t>());
+        hipLaunchKernelGGL(k_slice<0>, g, t, 0, c->stream, a, S, so, eo, bsl.as<uint8_t>(), bex.as<int32_t>());
     }
   }
   SA_CHECK_HIP(hipGetLastError());
-  return Sliced{bsl.as<uint16_t>(), bso.as<int64_t>(), bex.as<int32_t>(), beo.as<int64_t>()};
+  return Sliced{bsl.as<uint8_t>(), bso.as<int64_t>(), bex.as<int32_t>(), beo.as<int64_t>()};
 }
 
 static constexpr int kNumOzCfgs = (int)(sizeof(kOzCfgs) / sizeof(kOzCfgs[0]));
@@ -749,13 +783,13 @@ static double padded_ratio(const std::vector<int32_t>& rsz, const std::vector<in
 // Kernel choice: a forced table id that runs this slice count/type, else the
 // lowest padded work weighted by the relative cost per useful flop of each
 // kernel shape (deep 128-wide tiles < 64x64 < 32x32).
-static int pick_cfg(int S, bool bf, const sa_bsm* C, int tile) {
-  if (tile >= 0 && tile < kNumOzCfgs && kOzCfgs[tile].S == S && kOzCfgs[tile].bf == bf) return tile;
+static int pick_cfg(int S, int kind, const sa_bsm* C, int tile) {
+  if (tile >= 0 && tile < kNumOzCfgs && kOzCfgs[tile].S == S && kOzCfgs[tile].kind == kind) return tile;
   int best = -1;
   double best_cost = 1e300;
   for (int i = 0; i < kNumOzCfgs; ++i) {
     const OzCfg& k = kOzCfgs[i];
-    if (k.S != S || k.bf != bf) continue;
+    if (k.S != S || k.kind != kind) continue;
     const double w = k.deep ? 0.7 : (k.fm == 2 ? 1.0 : 1.6);
     const double cost = w * padded_ratio(C->rsz, C->csz, 32 * k.fm, 32 * k.fn);
     if (cost < best_cost) {
@@ -769,17 +803,20 @@ static int pick_cfg(int S, bool bf, const sa_bsm* C, int tile) {
 int64_t ozaki_multiply(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, double thresh, double alpha,
                        const sa_opts_t& o) {
   sa_context* c = C->ctx;
-  const bool bf = o.slice_type == SA_SLICE_BF16;
+  const int kind = o.slice_type;
+  SA_REQUIRE(kind == SA_SLICE_FP16 || kind == SA_SLICE_BF16 || kind == SA_SLICE_INT8, "ozaki: unknown slice type %d",
+             kind);
   const int S = o.slices;
-  const int cfg = pick_cfg(S, bf, C, -1);
-  SA_REQUIRE(cfg >= 0, "ozaki: %d %s slices not supported (FP16: 3..5, BF16: 4..6)", S, bf ? "BF16" : "FP16");
+  const int cfg = pick_cfg(S, kind, C, -1);
+  SA_REQUIRE(cfg >= 0, "ozaki: %d %s slices not supported (FP16: 3..5, BF16: 4..6, INT8: 4..6)", S,
+             kind == 2 ? "INT8" : kind == 1 ? "BF16" : "FP16");
   const ScreenResult R = screen_and_allocate(C, A, ta, B, tb, thresh, nullptr, nullptr, 0, false);
   if (R.ntasks == 0) return 0;
   // slice operands; share when both roles read the same vectors of one matrix
   const sa_bsm* ra = A->base ? A->base : A;
   const sa_bsm* rb = B->base ? B->base : B;
-  const Sliced sa_ = slice_operand(c, A, false, ta, S, bf, "oz_a");
-  const Sliced sb_ = (ra == rb && ta != tb && A == B) ? sa_ : slice_operand(c, B, true, tb, S, bf, "oz_b");
+  const Sliced sa_ = slice_operand(c, A, false, ta, S, kind, "oz_a");
+  const Sliced sb_ = (ra == rb && ta != tb && A == B) ? sa_ : slice_operand(c, B, true, tb, S, kind, "oz_b");
 
   DevBuf& bouts = c->buf("mm_outs");
   DevBuf& bprods = c->buf("oz_prods");
@@ -796,13 +833,14 @@ int64_t ozaki_multiply(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, double
   t.ta = ta; t.tb = tb;
   t.asl = sa_.sl; t.asl_off = sa_.sl_off; t.aex = sa_.ex; t.aex_off = sa_.ex_off;
   t.bsl = sb_.sl; t.bsl_off = sb_.sl_off; t.bex = sb_.ex; t.bex_off = sb_.ex_off;
+  t.kstep = oz_kstep(kind);
   hipLaunchKernelGGL(k_oz_records, dim3((unsigned)ceil_div(R.nc, 128)), dim3(128), 0, c->stream, t,
                      bouts.as<GemmOut>(), bprods.as<OzProd>());
 
   int32_t mm = 0, mn = 0;
   for (int32_t v : C->rsz) mm = std::max(mm, v);
   for (int32_t v : C->csz) mn = std::max(mn, v);
-  const int cf = pick_cfg(S, bf, C, o.tile);  // a forced id that runs other slices is ignored
+  const int cf = pick_cfg(S, kind, C, o.tile);  // a forced id that runs other slices is ignored
   const OzCfg& k = kOzCfgs[cf];
   const int TM = 32 * k.fm, TN = 32 * k.fn;
   const KWindows W = plan_kwindows(c, A, ta, B, tb, R, bouts.as<GemmOut>());

@@ -34,7 +34,8 @@ def run(oA, dA, ta, oB, dB, tb, oC, dC, thresh, opts, alpha=1.0, beta=0.0):
 
 
 @pytest.mark.parametrize("slices,stype", [(4, sa.SLICE_FP16), (5, sa.SLICE_FP16), (5, sa.SLICE_BF16),
-                                          (6, sa.SLICE_BF16)])
+                                          (6, sa.SLICE_BF16), (4, sa.SLICE_INT8), (5, sa.SLICE_INT8),
+                                          (6, sa.SLICE_INT8)])
 def test_config1_like_error_per_block_row(slices, stype):
     cfg = synth.config1(n=1024, bs=64)
     a = synth.dense_from_scales(1, cfg.rsz, cfg.ksz, cfg.a_scale)
@@ -46,7 +47,7 @@ def test_config1_like_error_per_block_row(slices, stype):
                                                                                slice_type=stype))
     err = rel_fro(got, ref)
     rows = block_row_errors(got, ref, cfg.rsz)
-    bound = TOL if (stype == sa.SLICE_FP16 or slices >= 5) else 1e-8
+    bound = TOL if (stype == sa.SLICE_FP16 or slices >= 5) else 1e-8  # 4 BF16 / INT8 slices: 30-31 bits
     assert err < bound, err
     assert rows.max() < bound, rows.max()
 
@@ -67,7 +68,7 @@ def test_uneven_transposes(ta, tb):
     assert rel_fro(got, ref) < TOL
 
 
-@pytest.mark.parametrize("stype,slices", [(sa.SLICE_FP16, 4), (sa.SLICE_BF16, 5)])
+@pytest.mark.parametrize("stype,slices", [(sa.SLICE_FP16, 4), (sa.SLICE_BF16, 5), (sa.SLICE_INT8, 5)])
 def test_deep_k_fold_schedule(stype, slices):
     """Products with k = 1000 and 1024 (32 k-steps, one ragged) chain over
     several fold intervals and product boundaries of one output block."""
@@ -85,7 +86,9 @@ def test_deep_k_fold_schedule(stype, slices):
 
 DEEP = [(12, 3, sa.SLICE_FP16), (13, 4, sa.SLICE_FP16), (14, 5, sa.SLICE_FP16), (15, 3, sa.SLICE_FP16),
         (16, 4, sa.SLICE_FP16), (17, 5, sa.SLICE_FP16), (18, 4, sa.SLICE_BF16), (19, 5, sa.SLICE_BF16),
-        (20, 4, sa.SLICE_BF16), (21, 5, sa.SLICE_BF16)]
+        (20, 4, sa.SLICE_BF16), (21, 5, sa.SLICE_BF16), (22, 4, sa.SLICE_INT8), (23, 5, sa.SLICE_INT8),
+        (24, 6, sa.SLICE_INT8), (25, 4, sa.SLICE_INT8), (26, 5, sa.SLICE_INT8), (28, 4, sa.SLICE_INT8),
+        (29, 5, sa.SLICE_INT8), (30, 4, sa.SLICE_INT8), (31, 5, sa.SLICE_INT8)]
 
 
 @pytest.mark.parametrize("tile,slices,stype", DEEP)
@@ -106,11 +109,12 @@ def test_deep_pipelined_kernels(tile, slices, stype, ta, tb):
     ref, got = run(oA, dA, ta, oB, dB, tb, oC, dC, 0.0,
                    sa.default_opts(mode=sa.MODE_OZAKI, slices=slices, slice_type=stype, tile=tile))
     # bars: 2^-(beta S - TS) truncation; 3 FP16 slices (28 bits) and 4 BF16 slices (31 bits) are coarser
-    bound = {(3, sa.SLICE_FP16): 1e-7, (4, sa.SLICE_BF16): 1e-8}.get((slices, stype), TOL)
+    bound = {(3, sa.SLICE_FP16): 1e-7, (4, sa.SLICE_BF16): 1e-8, (4, sa.SLICE_INT8): 1e-8}.get((slices, stype), TOL)
     assert rel_fro(got, ref) < bound
 
 
-def test_integer_inputs_exact():
+@pytest.mark.parametrize("stype", [sa.SLICE_FP16, sa.SLICE_BF16, sa.SLICE_INT8])
+def test_integer_inputs_exact(stype):
     """Inputs with at most 3 significant bits per row max live in the leading
     slice, so every dropped slice product is zero and the result is exact."""
     rng = np.random.default_rng(3)
@@ -119,8 +123,8 @@ def test_integer_inputs_exact():
     oA, dA = pair_uniform(a, 16, 16, 0.0)
     oB, dB = pair_uniform(b, 16, 16, 0.0)
     oC, dC = empty_pair_uniform(96, 112, 16, 16)
-    ref, got = run(oA, dA, 0, oB, dB, 0, oC, dC, 0.0, sa.default_opts(mode=sa.MODE_OZAKI, slices=4),
-                   alpha=1.0, beta=0.0)
+    ref, got = run(oA, dA, 0, oB, dB, 0, oC, dC, 0.0,
+                   sa.default_opts(mode=sa.MODE_OZAKI, slices=4, slice_type=stype), alpha=1.0, beta=0.0)
     assert np.array_equal(got, a @ b)
 
 
