@@ -111,9 +111,13 @@ def bench_rik(args):
     dBp = sa.BSM(ctx, w.p_sizes, bcs, 0.0)
     dK = sa.BSM(ctx, w.mu_sizes, w.mu_sizes, 0.0)
     o64 = sa.default_opts()
-    st = {"fp16": sa.SLICE_FP16, "bf16": sa.SLICE_BF16, "int8": sa.SLICE_INT8}[args.slice_type]
-    oz_t = sa.default_opts(mode=sa.MODE_OZAKI, slices=args.slices_transform, slice_type=st)
-    oz_k = sa.default_opts(mode=sa.MODE_OZAKI, slices=args.slices_k, slice_type=st)
+    kinds = {"fp16": sa.SLICE_FP16, "bf16": sa.SLICE_BF16, "int8": sa.SLICE_INT8}
+    forms = []
+    for f in args.forms.split(","):
+        parts = f.split(":")
+        kind, s_t, s_k = parts[0], int(parts[1]), int(parts[2])
+        tiles = (int(parts[3]), int(parts[4])) if len(parts) == 5 else (-1, -1)
+        forms.append((kind, s_t, s_k, tiles))
 
     # algorithmic flops from the task lists (transform, then K)
     log(f"B generated in HBM ({dB.info()[5] / 1e9:.1f} GB)")
@@ -125,9 +129,6 @@ def bench_rik(args):
     def step64():
         sa.ri_k_step(dX, dB, dBp, dK, w.mu_sizes, w.i_chunks, w.thresh, o64, o64)
 
-    def stepmix():
-        sa.ri_k_step(dX, dB, dBp, dK, w.mu_sizes, w.i_chunks, w.thresh, oz_t, oz_k)
-
     step64()
     ctx.synchronize()
     t_k = ctx.tasklist()
@@ -135,7 +136,6 @@ def bench_rik(args):
     kr = np.array([p * i for i in w.i_chunks for p in w.p_sizes])
     f_k = tasks_flops(t_k, w.mu_sizes, kr, w.mu_sizes)
     K64 = dK.to_dense()
-    Bp64_norm = dBp.frobenius_norm()
     for _ in range(max(args.warmup - 1, 0)):
         step64()
     ctx.set_profiling(True)
@@ -145,29 +145,39 @@ def bench_rik(args):
     cfg64 = ctx.gemm_config()[1]
     st64 = {k: round(ctx.profile(k)[0] / args.steps, 3) for k in ("rik_transform", "rik_k")}
     ctx.profile_reset()
-
-    for _ in range(args.warmup):
-        stepmix()
-    log("split-precision warm-up done")
-    msmix = timed(ctx, stepmix, args.steps)
-    log(f"split-precision step: {msmix:.1f} ms")
-    goz_ms, goz_n = ctx.profile("gemm_ozaki")
-    gsl_ms, _ = ctx.profile("ozaki_slice")
-    scr_ms, _ = ctx.profile("screen")
-    nrm_ms, _ = ctx.profile("norms")
-    stmix = {k: round(ctx.profile(k)[0] / args.steps, 3) for k in ("rik_transform", "rik_k")}
-    ctx.set_profiling(False)
-    Kmix = dK.to_dense()
-    err_k = rel(Kmix, K64)
-
     dense_fl = 2.0 * w.n_aux ** 2 * w.n_bf * w.n_occ + 2.0 * w.n_bf ** 2 * w.n_aux * w.n_occ
-    pairs_t = args.slices_transform * (args.slices_transform + 1) // 2
-    pairs_k = args.slices_k * (args.slices_k + 1) // 2
-    mfma_fl = pairs_t * f_tr + pairs_k * f_k       # low-precision MFMA work per step
-    oz_per_step_ms = goz_ms / args.steps
-    achieved_lp = mfma_fl / (oz_per_step_ms * 1e-3) / 1e12
-    peak_lp = PEAK_INT8 if args.slice_type == "int8" else PEAK_FP16
     fp64_achieved = (f_tr + f_k) / (g64_ms / args.steps * 1e-3) / 1e12
+
+    results = {}
+    for kind, s_t, s_k, tiles in forms:
+        name = f"{kind}_{s_t}_{s_k}" + ("" if tiles == (-1, -1) else f"_t{tiles[0]}_{tiles[1]}")
+        oz_t = sa.default_opts(mode=sa.MODE_OZAKI, slices=s_t, slice_type=kinds[kind], tile=tiles[0])
+        oz_k = sa.default_opts(mode=sa.MODE_OZAKI, slices=s_k, slice_type=kinds[kind], tile=tiles[1])
+
+        def stepmix():
+            sa.ri_k_step(dX, dB, dBp, dK, w.mu_sizes, w.i_chunks, w.thresh, oz_t, oz_k)
+
+        for _ in range(args.warmup):
+            stepmix()
+        ms = timed(ctx, stepmix, args.steps)
+        log(f"split-precision step ({name}): {ms:.1f} ms")
+        goz_ms, goz_n = ctx.profile("gemm_ozaki")
+        prof = {k: round(ctx.profile(k)[0] / args.steps, 3) for k in ("gemm_ozaki", "ozaki_slice", "screen", "norms",
+                                                                       "rik_transform", "rik_k")}
+        err = rel(dK.to_dense(), K64)
+        pairs = s_t * (s_t + 1) // 2 * f_tr + s_k * (s_k + 1) // 2 * f_k
+        peak = PEAK_INT8 if kind == "int8" else PEAK_FP16
+        ach = pairs / (goz_ms / args.steps * 1e-3) / 1e12
+        results[name] = {"slice_type": kind, "slices_transform": s_t, "slices_k": s_k, "step_ms": round(ms, 3),
+                         "speedup_vs_fp64_gpu": round(ms64 / ms, 3), "rel_err_K_vs_fp64": err,
+                         "meets_1e-10": bool(err <= 1e-10), "ms_per_step": prof,
+                         "lowp_tops": round(ach, 3), "lowp_peak": peak, "lowp_frac": round(ach / peak, 4),
+                         "kernel_avg_ms": round(goz_ms / max(goz_n, 1), 3)}
+    ctx.set_profiling(False)
+    ok = [k for k, v in results.items() if v["meets_1e-10"]]
+    head = min(ok or results, key=lambda k: results[k]["step_ms"])
+    H = results[head]
+    msmix = H["step_ms"]
 
     rec = {
         "metric": METRIC,
@@ -176,32 +186,28 @@ def bench_rik(args):
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(msmix, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": f"f64 with {args.slice_type}-slice products",
+        "dtype": f"f64 with {H['slice_type']}-slice products",
         "data": "synthetic (seeded geometry; B generated in HBM from a counter-based hash)",
         "config": {"workload": w.name, "natoms": w.natoms, "n_bf": w.n_bf, "n_aux": w.n_aux, "n_occ": w.n_occ,
                    "p_groups": int(len(w.p_sizes)), "mu_groups": int(len(w.mu_sizes)),
                    "i_chunks": [int(x) for x in w.i_chunks], "b_block_fill": round(float((w.b_scale > 0).mean()), 4),
-                   "thresh": w.thresh, "slices_transform": args.slices_transform, "slices_k": args.slices_k,
-                   "slice_type": args.slice_type, "parallelism": "single GPU"},
-        "ri_k": {"step_ms_split_precision": round(msmix, 3), "step_ms_fp64": round(ms64, 3),
-                 "speedup_vs_fp64_gpu": round(ms64 / msmix, 3), "rel_err_K_vs_fp64": err_k,
+                   "thresh": w.thresh, "form": head, "parallelism": "single GPU"},
+        "ri_k": {"step_ms_split_precision": msmix, "step_ms_fp64": round(ms64, 3),
+                 "speedup_vs_fp64_gpu": H["speedup_vs_fp64_gpu"], "rel_err_K_vs_fp64": H["rel_err_K_vs_fp64"],
                  "dense_equiv_tflops_fp64": round(dense_fl / (ms64 * 1e-3) / 1e12, 3),
                  "transform_tasks": int(len(t_tr)), "k_tasks": int(len(t_k)),
                  "algorithmic_flop": f_tr + f_k, "dense_equiv_flop": dense_fl,
                  "fp64_gemm_tflops": round(fp64_achieved, 3), "fp64_gemm_tile": cfg64,
-                 "stage_ms_fp64": st64, "stage_ms_split_precision": stmix,
-                 "split_ms_per_step": {"gemm_ozaki": round(oz_per_step_ms, 3),
-                                       "slicing": round(gsl_ms / args.steps, 3),
-                                       "screening": round(scr_ms / args.steps, 3),
-                                       "norms": round(nrm_ms / args.steps, 3)},
+                 "stage_ms_fp64": st64, "forms": results,
                  "setup": {"metric_host_s": round(t_metric, 3), "inv_sqrt_s": round(t_ns, 3),
                            "newton_schulz_iterations": ns_it, "newton_schulz_residual": ns_res}},
-        "roofline": {"bound": "mfma", "achieved": round(achieved_lp, 3), "peak": peak_lp,
-                     "unit": "TOP/s" if args.slice_type == "int8" else "TFLOP/s",
-                     "frac": round(achieved_lp / peak_lp, 4), "traffic": None, "kernel": "k_gemm_ozaki",
-                     "kernel_avg_ms": round(goz_ms / max(goz_n, 1), 3),
-                     "note": "achieved = executed FP16 slice-product flops (pairs x algorithmic) / kernel time; "
-                             "FP64 path k_gemm_fp64 reported in ri_k.fp64_gemm_tflops (peak 78.6)"},
+        "roofline": {"bound": "mfma", "achieved": H["lowp_tops"], "peak": H["lowp_peak"],
+                     "unit": "TOP/s" if H["slice_type"] == "int8" else "TFLOP/s",
+                     "frac": H["lowp_frac"], "traffic": None, "kernel": "k_gemm_ozaki",
+                     "kernel_avg_ms": H["kernel_avg_ms"],
+                     "note": "achieved = executed low-precision slice-product operations (pairs x algorithmic "
+                             "flop) / k_gemm_ozaki time over the timed steps; the FP64 path's k_gemm_fp64 is in "
+                             "ri_k.fp64_gemm_tflops (peak 78.6)"},
     }
     if not args.no_cpu:
         log("CPU baseline sample (oracle) ...")
@@ -370,9 +376,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="rik", choices=["rik", "config1", "config2", "config3", "config5"])
     ap.add_argument("--tile", type=int, default=-1)
-    ap.add_argument("--slices-transform", type=int, default=4)
-    ap.add_argument("--slices-k", type=int, default=4)
-    ap.add_argument("--slice-type", default="fp16", choices=["fp16", "bf16", "int8"])
+    ap.add_argument("--slices-k", type=int, default=4, help="split-precision slices (config3 workload)")
+    ap.add_argument("--forms", default="int8:5:5,fp16:4:4",
+                    help="RI-K split-precision forms kind:slices_transform:slices_k[:tile_transform:tile_k] "
+                         "(kind fp16|bf16|int8; tiles = split-precision kernel table ids, default chosen)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--n-occ", type=int, default=200, help="RI-K occupied orbitals (200 = the headline workload)")
     ap.add_argument("--no-check", action="store_true", help="config5: skip the learned-vs-heuristic comparison")

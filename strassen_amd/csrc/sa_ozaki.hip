@@ -782,14 +782,24 @@ static double padded_ratio(const std::vector<int32_t>& rsz, const std::vector<in
 
 // Kernel choice: a forced table id that runs this slice count/type, else the
 // lowest padded work weighted by the relative cost per useful flop of each
-// kernel shape (deep 128-wide tiles < 64x64 < 32x32).
+// kernel shape (deep 128-wide tiles < 64x64 < 32x32).  The deep kernels (one
+// 8-wave workgroup per CU) only pay on C blocks of at least 128 on both sides;
+// below that two independent 64x64 workgroups per CU hide latency better
+// (measured on the RI-K K contraction's ~100-wide blocks: 8 % faster).
 static int pick_cfg(int S, int kind, const sa_bsm* C, int tile) {
   if (tile >= 0 && tile < kNumOzCfgs && kOzCfgs[tile].S == S && kOzCfgs[tile].kind == kind) return tile;
+  auto mean = [](const std::vector<int32_t>& v) {
+    double t = 0;
+    for (int32_t x : v) t += x;
+    return v.empty() ? 0.0 : t / (double)v.size();
+  };
+  const bool large = std::min(mean(C->rsz), mean(C->csz)) >= 128.0;
   int best = -1;
   double best_cost = 1e300;
   for (int i = 0; i < kNumOzCfgs; ++i) {
     const OzCfg& k = kOzCfgs[i];
     if (k.S != S || k.kind != kind) continue;
+    if (k.deep && !large) continue;
     const double w = k.deep ? 0.7 : (k.fm == 2 ? 1.0 : 1.6);
     const double cost = w * padded_ratio(C->rsz, C->csz, 32 * k.fm, 32 * k.fn);
     if (cost < best_cost) {
