@@ -16,10 +16,10 @@
 //  * digit products of one weight w = p + q accumulate in FP32 inside
 //    v_mfma_f32_16x16x32_{f16,bf16}; the sums of the weights that carry the
 //    leading bits stay exact (integers below 2^24) between two folds into the
-//    FP64 accumulators: FP16 weight 0 for 8 k-steps of 32 (256^2*32*8 = 2^24),
-//    weight 1 for 2 (2*256*512*32*2 = 2^24); BF16 weights 0..2 for 8.  Later
-//    weights are folded on the same schedule and may round in FP32, which
-//    costs at most 2^-(2 beta + 21) of the leading term per fold.
+//    FP64 accumulators: FP16 weights 0..1 for 2 k-steps of 32 (weight 1:
+//    2*256*512*32*2 = 2^24), BF16 weights 0..2 for 8, so all weights are
+//    folded every 2 (FP16) or 8 (BF16) k-steps.  Later weights may round in
+//    FP32, which costs at most 2^-(2 beta + 21) of the leading term per fold.
 // INT8 slices (an MI355X-first third form, v_mfma_i32_16x16x64_i8 at twice
 // the FP16 rate): beta = 8, TS = 2 digits in int8; digit products accumulate
 // in int32, exact for 2^11 / S k-steps of 64 (|d_p d_q| <= 2^14), so products
@@ -52,8 +52,9 @@ __host__ __device__ constexpr int oz_beta(int kind) { return kind == 0 ? 10 : 8;
 __host__ __device__ constexpr int oz_tshift(int kind) { return kind == 1 ? 1 : 2; }
 __host__ __device__ constexpr int oz_kstep(int kind) { return kind == 2 ? 64 : 32; }  // MFMA k
 __host__ __device__ constexpr int oz_esize(int kind) { return kind == 2 ? 1 : 2; }    // bytes per digit
-// k-steps between folds of all weights (FP16 also folds weights 1..2 every 2)
-__host__ __device__ constexpr int oz_full_fold(int kind) { return kind == 2 ? 256 : 8; }
+// k-steps between folds of all weights into FP64 (FP16 weight 1 is exact for 2
+// k-steps of 32, BF16 weights 0..2 for 8, INT8 all weights for 256 of 64)
+__host__ __device__ constexpr int oz_full_fold(int kind) { return kind == 2 ? 256 : kind == 1 ? 8 : 2; }
 
 template <int KIND>
 struct OzAcc {
@@ -262,15 +263,41 @@ __global__ void k_oz_records(OzTables t, GemmOut* __restrict__ outs, OzProd* __r
 }
 
 // ----------------------------------------------------------------- GEMM --
+// Wave-wide 16-byte load from global memory straight into LDS: lane l's 16
+// bytes land at lds + 16 l (lds wave-uniform); completion is counted on vmcnt.
+__device__ inline void lds_dma16(const uint4* src, uint4* lds) {
+  __builtin_amdgcn_global_load_lds((const void*)src, (__attribute__((address_space(3))) void*)lds, 16, 0, 0);
+}
+
+// Calls f(NA, NB) with the wave's live fragment counts as compile-time
+// constants: (FM, FN), the partial shapes, or (0, 0) for a wave with no live
+// fragment (it still takes its share of the loads and every barrier).
+// FM, FN <= 2.
+template <int FM, int FN, class F>
+__device__ inline void oz_dispatch(int na, int nb, F&& f) {
+  static_assert(FM >= 1 && FM <= 2 && FN >= 1 && FN <= 2, "fragment counts per wave");
+  using C0 = std::integral_constant<int, 0>;
+  using C1 = std::integral_constant<int, 1>;
+  using CM = std::integral_constant<int, FM>;
+  using CN = std::integral_constant<int, FN>;
+  if (na >= FM && nb >= FN) f(CM{}, CN{});
+  else if (FN > 1 && na >= FM && nb == 1) f(CM{}, C1{});
+  else if (FM > 1 && na == 1 && nb >= FN) f(C1{}, CN{});
+  else if (FM > 1 && FN > 1 && na == 1 && nb == 1) f(C1{}, C1{});
+  else f(C0{}, C0{});
+}
+
 // One workgroup of 4 waves (2 x 2) owns a (32 FM) x (32 FN) output tile; each
-// wave owns (16 FM) x (16 FN).  A stage is one k-step of 32 for all S slices of
+// wave owns (16 FM) x (16 FN).  A stage is one MFMA k-step for all S slices of
 // the tile's op(A) rows and op(B) columns, i.e. S (2 FM + 2 FN) fragments of
-// 1 KB, double-buffered in LDS.  The stream of stages runs over the products
-// of the output; the global loads of stage s+1 are in flight while the MFMAs of
-// stage s run, with one barrier per stage.  Fragments of 16-row blocks beyond
-// the output block hold zeros (their loads are skipped) and run through the
-// MFMAs like the others: a branch per fragment would put every MFMA in its own
-// basic block and serialise it behind the LDS reads.
+// 1 KB, double-buffered in LDS and loaded straight from global memory into
+// LDS (no register staging).  The stream of stages runs over the products of
+// the output; the loads of stage s+1 are in flight while the MFMAs of stage s
+// run, with one barrier per stage.  Fragments beyond the output block are
+// neither loaded nor multiplied: the whole stage loop is specialised on the
+// wave's live fragment counts, so each specialisation issues its MFMAs as one
+// straight-line block (a branch per fragment would put every MFMA in its own
+// basic block and serialise it behind the LDS reads).
 template <int S, int KIND, int FM, int FN>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
     k_gemm_ozaki(const GemmOut* __restrict__ outs, const OzProd* __restrict__ prods,
@@ -280,10 +307,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
   constexpr int RA = 2 * FM, RB = 2 * FN;    // 16-blocks of rows / columns per tile
   constexpr int NCA = S * RA, NCB = S * RB;  // fragments per stage
   constexpr int LA = (NCA + 3) / 4, LB = (NCB + 3) / 4;
-  constexpr int FREQ_LO = 1, FREQ_HI = (KIND != 0 || S < 3) ? 1 : 3;  // weights folded every 2 k-steps (FP16)
   constexpr int FULL = oz_full_fold(KIND);
-  __shared__ uint4 As[2][NCA][64];
-  __shared__ uint4 Bs[2][NCB][64];
+  // the two stage buffers are separate objects: the compiler then sees that
+  // the LDS reads of one stage do not wait for the loads into the other
+  __shared__ uint4 As0[NCA][64], As1[NCA][64];
+  __shared__ uint4 Bs0[NCB][64], Bs1[NCB][64];
 
   const int tid = threadIdx.x, lane = tid & 63;
   const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
@@ -295,26 +323,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
   const int m = o.m, n = o.n;
   const int nrb = (int)ceil_div(m, 16), ncb = (int)ceil_div(n, 16);
   const int rb0 = item.tm * RA, cb0 = item.tn * RB;
-  bool act_i[FM], act_j[FN];
-#pragma unroll
-  for (int i = 0; i < FM; ++i) act_i[i] = rb0 + wm * FM + i < nrb;
-#pragma unroll
-  for (int j = 0; j < FN; ++j) act_j[j] = cb0 + wn * FN + j < ncb;
+  const int nai = __builtin_amdgcn_readfirstlane(nrb - rb0 - wm * FM);  // live fragment rows (may be <= 0)
+  const int naj = __builtin_amdgcn_readfirstlane(ncb - cb0 - wn * FN);
 
-  dbl4 acc[FM][FN];
-  acc4 s32[S][FM][FN];
-#pragma unroll
-  for (int i = 0; i < FM; ++i)
-#pragma unroll
-    for (int j = 0; j < FN; ++j) {
-      acc[i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-      for (int w = 0; w < S; ++w) s32[w][i][j] = acc4{0, 0, 0, 0};
-    }
-  int er[FM][4], ec[FN];
-  uint4 ra[LA], rbf[LB];
-
-  auto gload = [&](const OzProd& pr, int ks) {
+  // stage (product pr, k-step ks) into LDS buffer buf: one 1 KB fragment per
+  // wave-wide 16-byte load; fragments beyond the output block are skipped
+  auto gload = [&](const OzProd& pr, int ks, auto par) {
+    uint4(*As)[64] = decltype(par)::value ? As1 : As0;
+    uint4(*Bs)[64] = decltype(par)::value ? Bs1 : Bs0;
     const uint4* a4 = reinterpret_cast<const uint4*>(pr.a);
     const uint4* b4 = reinterpret_cast<const uint4*>(pr.b);
     const int64_t psa = (int64_t)pr.nvba * pr.nks * 64, psb = (int64_t)pr.nvbb * pr.nks * 64;
@@ -322,143 +338,145 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
     for (int r = 0; r < LA; ++r) {
       const int c = wid + 4 * r;
       const int p = c / RA, vb = rb0 + c % RA;
-      ra[r] = (c < NCA && vb < nrb) ? gload1(a4 + p * psa + ((int64_t)vb * pr.nks + ks) * 64 + lane)
-                                    : make_uint4(0, 0, 0, 0);
+      if (c < NCA && vb < nrb) lds_dma16(a4 + p * psa + ((int64_t)vb * pr.nks + ks) * 64 + lane, &As[c][0]);
     }
 #pragma unroll
     for (int r = 0; r < LB; ++r) {
       const int c = wid + 4 * r;
       const int p = c / RB, vb = cb0 + c % RB;
-      rbf[r] = (c < NCB && vb < ncb) ? gload1(b4 + p * psb + ((int64_t)vb * pr.nks + ks) * 64 + lane)
-                                     : make_uint4(0, 0, 0, 0);
+      if (c < NCB && vb < ncb) lds_dma16(b4 + p * psb + ((int64_t)vb * pr.nks + ks) * 64 + lane, &Bs[c][0]);
     }
   };
-  auto lstore = [&](int buf) {
+
+  // the stage loop and write-back for a wave with NA x NB live fragments
+  auto run = [&](auto na_, auto nb_) {
+    constexpr int NA = decltype(na_)::value, NB = decltype(nb_)::value;
+    constexpr int EA = NA > 0 ? NA : 1, EB = NB > 0 ? NB : 1;  // array extents
+    dbl4 acc[EA][EB];
+    acc4 s32[S][EA][EB];
 #pragma unroll
-    for (int r = 0; r < LA; ++r) {
-      const int c = wid + 4 * r;
-      if (c < NCA) As[buf][c][lane] = ra[r];
-    }
+    for (int i = 0; i < EA; ++i)
 #pragma unroll
-    for (int r = 0; r < LB; ++r) {
-      const int c = wid + 4 * r;
-      if (c < NCB) Bs[buf][c][lane] = rbf[r];
-    }
-  };
-  auto load_exps = [&](const OzProd& pr) {
+      for (int j = 0; j < EB; ++j) {
+        acc[i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int i = 0; i < FM; ++i)
+        for (int w = 0; w < S; ++w) s32[w][i][j] = acc4{0, 0, 0, 0};
+      }
+    int er[EA][4], ec[EB];
+    auto load_exps = [&](const OzProd& pr) {
 #pragma unroll
-      for (int v = 0; v < 4; ++v) er[i][v] = act_i[i] ? gload1(pr.ea + (rb0 + wm * FM + i) * 16 + 4 * lg + v) : 0;
+      for (int i = 0; i < NA; ++i)
 #pragma unroll
-    for (int j = 0; j < FN; ++j) ec[j] = act_j[j] ? gload1(pr.eb + (cb0 + wn * FN + j) * 16 + li) : 0;
-  };
-  auto compute = [&](int buf) {
-    uint4 bq[S][FN];
+        for (int v = 0; v < 4; ++v) er[i][v] = gload1(pr.ea + (rb0 + wm * FM + i) * 16 + 4 * lg + v);
 #pragma unroll
-    for (int q = 0; q < S; ++q)
+      for (int j = 0; j < NB; ++j) ec[j] = gload1(pr.eb + (cb0 + wn * FN + j) * 16 + li);
+    };
+    auto compute = [&](auto par) {
+      const uint4(*As)[64] = decltype(par)::value ? As1 : As0;
+      const uint4(*Bs)[64] = decltype(par)::value ? Bs1 : Bs0;
+      uint4 bq[S][EB];
 #pragma unroll
-      for (int j = 0; j < FN; ++j) bq[q][j] = Bs[buf][q * RB + wn * FN + j][lane];
+      for (int q = 0; q < S; ++q)
 #pragma unroll
-    for (int pa = 0; pa < S; ++pa) {
-      uint4 aq[FM];
+        for (int j = 0; j < NB; ++j) bq[q][j] = Bs[q * RB + wn * FN + j][lane];
 #pragma unroll
-      for (int i = 0; i < FM; ++i) aq[i] = As[buf][pa * RA + wm * FM + i][lane];
+      for (int pa = 0; pa < S; ++pa) {
+        uint4 aq[EA];
 #pragma unroll
-      for (int pb = 0; pb < S - pa; ++pb)
+        for (int i = 0; i < NA; ++i) aq[i] = As[pa * RA + wm * FM + i][lane];
 #pragma unroll
-        for (int i = 0; i < FM; ++i)
+        for (int pb = 0; pb < S - pa; ++pb)
 #pragma unroll
-          for (int j = 0; j < FN; ++j) {
-            s32[pa + pb][i][j] = oz_mfma(std::integral_constant<int, KIND>{}, aq[i], bq[pb][j], s32[pa + pb][i][j]);
+          for (int i = 0; i < NA; ++i)
+#pragma unroll
+            for (int j = 0; j < NB; ++j)
+              s32[pa + pb][i][j] = oz_mfma(std::integral_constant<int, KIND>{}, aq[i], bq[pb][j], s32[pa + pb][i][j]);
+      }
+    };
+    // acc += sum_w s_w 2^(e_r + f_c + 2 TS - beta (w + 2)), s = 0: t = sum_w
+    // s_w 2^(-beta w) by Horner steps, one scaling per element
+    auto fold = [&]() {
+#pragma unroll
+      for (int i = 0; i < NA; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            double t = (double)s32[S - 1][i][j][v];
+#pragma unroll
+            for (int w = S - 2; w >= 0; --w) t = fma(t, pow2i(-BETA), (double)s32[w][i][j][v]);
+            acc[i][j][v] += ldexp(t, er[i][v] + ec[j] + 2 * TS - 2 * BETA);
           }
-    }
-  };
-  // acc += sum_{w in [W0, W1)} s_w 2^(e_r + f_c + 2 TS - beta (w + 2)); s_w = 0
-  // acc += s_w 2^(e_r + f_c + 2 TS - beta (w + 2)) and s_w = 0 for w in [lo, hi)
-  // (lo, hi wave-uniform; one call site, so no weight is ever indexed at run time)
-  auto fold = [&](int lo, int hi) {
 #pragma unroll
-    for (int w = 0; w < S; ++w) {
-      if (w < lo || w >= hi) continue;
-#pragma unroll
-      for (int i = 0; i < FM; ++i)
-#pragma unroll
-        for (int j = 0; j < FN; ++j) {
-#pragma unroll
-          for (int v = 0; v < 4; ++v)
-            acc[i][j][v] += ldexp((double)s32[w][i][j][v], er[i][v] + ec[j] + 2 * TS - BETA * (w + 2));
-          s32[w][i][j] = acc4{0, 0, 0, 0};
+          for (int w = 0; w < S; ++w) s32[w][i][j] = acc4{0, 0, 0, 0};
         }
-    }
-  };
+    };
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
 
-  int p = o.p0;
-  if (p < o.p1) {
-    OzProd pr = gload1(prods + p);
-    // record of the following product, loaded one product ahead
-    OzProd pf = p + 1 < o.p1 ? gload1(prods + p + 1) : pr;
-    int ks = 0;
-    load_exps(pr);
-    gload(pr, 0);
-    lstore(0);
-    __syncthreads();
-    int buf = 0;
-    while (true) {
-      int np = p, nks = ks + 1;
-      if (nks >= pr.nks) {
-        np = p + 1;
-        nks = 0;
-      }
-      const bool has_next = np < o.p1;
-      OzProd pn = pr;
-      if (has_next) {
-        if (np != p) {
-          pn = pf;
-          if (np + 1 < o.p1) pf = gload1(prods + np + 1);
-        }
-        gload(pn, nks);
-      }
-      compute(buf);
-      const int done = ks + 1;
-      int lo = S, hi = 0;
-      if (np != p || (done & (FULL - 1)) == 0) {
-        lo = 0;
-        hi = S;
-      } else if (FREQ_HI > FREQ_LO && (done & 1) == 0) {
-        lo = FREQ_LO;
-        hi = FREQ_HI;
-      }
-      if (lo < hi) fold(lo, hi);
-      if (!has_next) break;
-      if (np != p) load_exps(pn);
-      lstore(buf ^ 1);
+    int p = o.p0;
+    if (p < o.p1) {
+      OzProd pr = gload1(prods + p);
+      // record of the following product, loaded one product ahead
+      OzProd pf = p + 1 < o.p1 ? gload1(prods + p + 1) : pr;
+      int ks = 0;
+      load_exps(pr);
+      gload(pr, 0, I0{});
       __syncthreads();
-      buf ^= 1;
-      p = np;
-      ks = nks;
-      pr = pn;
-    }
-  }
-
-#pragma unroll
-  for (int i = 0; i < FM; ++i)
-#pragma unroll
-    for (int j = 0; j < FN; ++j) {
-      if (!(act_i[i] && act_j[j])) continue;
-      const int col = (cb0 + wn * FN + j) * 16 + li;
-      if (col >= n) continue;
-      double* cp = o.c + (int64_t)col * m;
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int row = (rb0 + wm * FM + i) * 16 + 4 * lg + v;
-        if (row < m) {
-          double x = alpha * acc[i][j][v];
-          if (o.acc) x = x + cp[row];
-          cp[row] = x;
+      // one stage computing from buffer P while the next loads into P ^ 1;
+      // true at the end of the stream
+      auto step = [&](auto par) -> bool {
+        constexpr int P = decltype(par)::value;
+        int np = p, nks = ks + 1;
+        if (nks >= pr.nks) {
+          np = p + 1;
+          nks = 0;
         }
+        const bool has_next = np < o.p1;
+        OzProd pn = pr;
+        if (has_next) {
+          if (np != p) {
+            pn = pf;
+            if (np + 1 < o.p1) pf = gload1(prods + np + 1);
+          }
+          gload(pn, nks, std::integral_constant<int, P ^ 1>{});
+        }
+        compute(par);
+        const int done = ks + 1;
+        if (np != p || (done & (FULL - 1)) == 0) fold();
+        if (!has_next) return true;
+        if (np != p) load_exps(pn);
+        __syncthreads();
+        p = np;
+        ks = nks;
+        pr = pn;
+        return false;
+      };
+      while (true) {
+        if (step(I0{})) break;
+        if (step(I1{})) break;
       }
     }
+
+#pragma unroll
+    for (int i = 0; i < NA; ++i)
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        const int col = (cb0 + wn * FN + j) * 16 + li;
+        if (col >= n) continue;
+        double* cp = o.c + (int64_t)col * m;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = (rb0 + wm * FM + i) * 16 + 4 * lg + v;
+          if (row < m) {
+            double x = alpha * acc[i][j][v];
+            if (o.acc) x = x + cp[row];
+            cp[row] = x;
+          }
+        }
+      }
+  };
+  oz_dispatch<FM, FN>(nai, naj, run);
 }
 
 // Deep-pipelined variant for large blocks: WM x WN waves (8: two per SIMD) of
@@ -484,7 +502,7 @@ __global__ void __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_
   constexpr int NCA = S * RA, NCB = S * RB;
   constexpr int LA = (NCA + NW - 1) / NW, LB = (NCB + NW - 1) / NW;
   // FP16: weight 1 is exact for 2 k-steps of 32, BF16 weights 0..2 for 8, INT8 all for 256 of 64
-  constexpr int FOLD_EVERY = KIND == 0 ? 2 : oz_full_fold(KIND);
+  constexpr int FOLD_EVERY = oz_full_fold(KIND);
   __shared__ uint4 As[2][NCA][64];
   __shared__ uint4 Bs[2][NCB][64];
 
