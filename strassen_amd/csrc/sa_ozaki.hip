@@ -133,59 +133,109 @@ __global__ void k_slice_need(SliceArgs s, int S, int kstep, int esize, int64_t* 
   need_ex[b] = pres ? nvp : 0;
 }
 
+// One single-wave workgroup per 16-vector group of a block (groups numbered
+// through the exclusive scan of the padded vector counts, ex_off).  Each
+// k-step of the group, a 16 x KS tile of the operand, is read with coalesced
+// loads (along k where k is contiguous, else along the vectors) into LDS; lane
+// l then owns vector 16 vb + (l & 15) and the EL digits at k = KS ks +
+// EL (l >> 4) + t, i.e. exactly lane l's 16 bytes of the fragment, so every
+// slice of a fragment is one coalesced 1 KB store.  Pass 1 reduces max|x| per
+// vector; pass 2 re-reads the tiles and cuts the values into digits.  Balanced
+// digits in closed form: with m = n + H, H = sum_{j < S-1} 2^(beta-1) 2^(beta j),
+// the digit at position j < S-1 is ((m >> beta j) & (2^beta - 1)) - 2^(beta-1)
+// and the leading one m >> beta (S-1), the same digits as peeling n from its
+// least significant end.
 template <int KIND>
-__global__ void __launch_bounds__(256) k_slice(SliceArgs s, int S, const int64_t* __restrict__ sl_off,
-                                               const int64_t* __restrict__ ex_off, uint8_t* __restrict__ slices,
-                                               int32_t* __restrict__ exps) {
+__global__ void __launch_bounds__(64) k_slice(SliceArgs s, int S, const int64_t* __restrict__ sl_off,
+                                              const int64_t* __restrict__ ex_off, int64_t ngroups,
+                                              uint8_t* __restrict__ slices, int32_t* __restrict__ exps) {
   constexpr int BETA = oz_beta(KIND), TS = oz_tshift(KIND);
   constexpr int KS = oz_kstep(KIND), ES = oz_esize(KIND), EL = 16 / ES;
-  const int64_t b = blockIdx.x;
-  const int64_t o = s.off[b];
-  if (o < 0) return;
+  __shared__ double T[16][KS + 1];  // one k-step tile; the pad keeps the 16 rows on distinct banks
+  const int lane = threadIdx.x;
+  const int64_t g = blockIdx.x;
+  if (g >= ngroups) return;
+  // block of group g: the last b with ex_off[b] <= 16 g (absent blocks have no
+  // vectors, so equal offsets end at the present block)
+  const int64_t key = 16 * g;
+  int64_t lo = 0, hi = s.nb;
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (ex_off[mid] <= key) lo = mid;
+    else hi = mid;
+  }
+  const int64_t b = lo;
   int nv, kd;
   int64_t sv, sk;
   slice_geom(s, b, nv, kd, sv, sk);
   const int nks = (int)ceil_div(kd, KS), kp = nks * KS;
-  const int nvb = (int)ceil_div(nv, 16), nvp = nvb * 16;
+  const int nvp = (int)ceil_div(nv, 16) * 16;
   const int64_t pstride = (int64_t)nvp * kp * ES;  // bytes per slice
-  const double* x = s.arena + o;
-  uint8_t* out = slices + sl_off[b];
-  int32_t* ex = exps + ex_off[b];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int64_t mask = ((int64_t)1 << BETA) - 1, half = (int64_t)1 << (BETA - 1);
-  for (int v = w; v < nvp; v += 4) {
-    double mx = 0.0;
-    if (v < nv)
-      for (int kk = lane; kk < kd; kk += 64) mx = fmax(mx, fabs(x[v * sv + kk * sk]));
+  const int vb = (int)((key - ex_off[b]) >> 4), v0 = vb * 16;
+  const double* xb = s.arena + s.off[b];
+  uint8_t* out = slices + sl_off[b] + (int64_t)vb * nks * 1024 + lane * 16;
+  const int vl = lane & 15, kg = lane >> 4;
+
+  // T[i][kl] = x(vector v0 + i, k = KS ks + kl), zero outside the block
+  auto stage = [&](int ks) {
+    const int k0 = ks * KS;
+    if (sk == 1) {  // k contiguous: each load covers 64 / KS vectors x KS k
+      constexpr int VPI = 64 / KS;
+      const int kl = lane % KS, vo = lane / KS;
 #pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
-    int e = 0;
-    if (mx > 0.0) (void)frexp(mx, &e);  // mx < 2^e
-    if (lane == 0) ex[v] = e;
-    const int64_t vbase = (int64_t)(v >> 4) * nks * 1024 + (v & 15) * 16;
-    for (int kk = lane; kk < kp; kk += 64) {
-      const double a = (v < nv && kk < kd) ? x[v * sv + kk * sk] : 0.0;
-      int64_t n = mx > 0.0 ? (int64_t)rint(ldexp(a, BETA * S - TS - e)) : 0;
-      const int64_t at = vbase + (int64_t)(kk / KS) * 1024 + ((kk % KS) / EL) * 256 + (kk % EL) * ES;
-      // balanced digits, least significant first
-      for (int p = S - 1; p >= 0; --p) {
-        int64_t d;
-        if (p > 0) {
-          d = n & mask;
-          if (d >= half) d -= (int64_t)1 << BETA;
-          n = (n - d) >> BETA;
-        } else {
-          d = n;
-        }
-        uint8_t* dst = out + (int64_t)p * pstride + at;
+      for (int j = 0; j < 16 / VPI; ++j) {
+        const int i = j * VPI + vo, v = v0 + i, kk = k0 + kl;
+        T[i][kl] = (v < nv && kk < kd) ? xb[(int64_t)v * sv + kk] : 0.0;
+      }
+    } else {  // vectors contiguous (or neither): each load covers 16 vectors x 4 k
+#pragma unroll
+      for (int j = 0; j < KS / 4; ++j) {
+        const int kl = j * 4 + kg, v = v0 + vl, kk = k0 + kl;
+        T[vl][kl] = (v < nv && kk < kd) ? xb[(int64_t)v * sv + (int64_t)kk * sk] : 0.0;
+      }
+    }
+  };
+
+  double mx = 0.0;
+  for (int ks = 0; ks < nks; ++ks) {
+    stage(ks);
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < EL; ++t) mx = fmax(mx, fabs(T[vl][kg * EL + t]));
+    __syncthreads();
+  }
+  mx = fmax(mx, __shfl_xor(mx, 16, 64));
+  mx = fmax(mx, __shfl_xor(mx, 32, 64));
+  int e = 0;
+  if (mx > 0.0) (void)frexp(mx, &e);  // mx < 2^e
+  if (kg == 0) exps[ex_off[b] + v0 + vl] = e;
+
+  const int64_t mask = ((int64_t)1 << BETA) - 1, half = (int64_t)1 << (BETA - 1);
+  int64_t H = 0;
+  for (int j = 0; j < S - 1; ++j) H += half << (BETA * j);
+  for (int ks = 0; ks < nks; ++ks) {
+    stage(ks);
+    __syncthreads();
+    int64_t mv[EL];
+#pragma unroll
+    for (int t = 0; t < EL; ++t)
+      mv[t] = (mx > 0.0 ? (int64_t)rint(ldexp(T[vl][kg * EL + t], BETA * S - TS - e)) : 0) + H;
+    __syncthreads();
+    for (int p = 0; p < S; ++p) {
+      const int sh = BETA * (S - 1 - p);
+      uint32_t w[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+      for (int t = 0; t < EL; ++t) {
+        const int64_t d = p == 0 ? (mv[t] >> sh) : ((mv[t] >> sh) & mask) - half;
         if (KIND == 2) {
-          *dst = (uint8_t)(int8_t)d;
+          w[t >> 2] |= ((uint32_t)d & 0xffu) << (8 * (t & 3));
         } else {
           const uint16_t bits = KIND == 1 ? __builtin_bit_cast(uint16_t, (__bf16)(float)d)
                                           : __builtin_bit_cast(uint16_t, (_Float16)(float)d);
-          *reinterpret_cast<uint16_t*>(dst) = bits;
+          w[t >> 1] |= (uint32_t)bits << (16 * (t & 1));
         }
       }
+      *reinterpret_cast<uint4*>(out + (int64_t)p * pstride + (int64_t)ks * 1024) = make_uint4(w[0], w[1], w[2], w[3]);
     }
   }
 }
@@ -773,14 +823,15 @@ static Sliced slice_operand(sa_context* c, sa_bsm* X, bool role_b, bool trans, i
   {
     ProfScope ps(c, "ozaki_slice");
     if (nb) {
-      const dim3 g((unsigned)nb), t(256);
+      const int64_t ng = nex / 16;  // 16-vector groups (nex: padded vectors of all blocks)
+      const dim3 g((unsigned)std::max<int64_t>(1, ng)), t(64);
       const int64_t *so = (const int64_t*)bso.p, *eo = (const int64_t*)beo.p;
       if (kind == 2)
-        hipLaunchKernelGGL(k_slice<2>, g, t, 0, c->stream, a, S, so, eo, bsl.as<uint8_t>(), bex.as<int32_t>());
+        hipLaunchKernelGGL(k_slice<2>, g, t, 0, c->stream, a, S, so, eo, ng, bsl.as<uint8_t>(), bex.as<int32_t>());
       else if (kind == 1)
-        hipLaunchKernelGGL(k_slice<1>, g, t, 0, c->stream, a, S, so, eo, bsl.as<uint8_t>(), bex.as<int32_t>());
+        hipLaunchKernelGGL(k_slice<1>, g, t, 0, c->stream, a, S, so, eo, ng, bsl.as<uint8_t>(), bex.as<int32_t>());
       else
-        hipLaunchKernelGGL(k_slice<0>, g, t, 0, c->stream, a, S, so, eo, bsl.as<uint8_t>(), bex.as<int32_t>());
+        hipLaunchKernelGGL(k_slice<0>, g, t, 0, c->stream, a, S, so, eo, ng, bsl.as<uint8_t>(), bex.as<int32_t>());
     }
   }
   SA_CHECK_HIP(hipGetLastError());
@@ -800,10 +851,12 @@ static double padded_ratio(const std::vector<int32_t>& rsz, const std::vector<in
 
 // Kernel choice: a forced table id that runs this slice count/type, else the
 // lowest padded work weighted by the relative cost per useful flop of each
-// kernel shape (deep 128-wide tiles < 64x64 < 32x32).  The deep kernels (one
-// 8-wave workgroup per CU) only pay on C blocks of at least 128 on both sides;
-// below that two independent 64x64 workgroups per CU hide latency better
-// (measured on the RI-K K contraction's ~100-wide blocks: 8 % faster).
+// kernel shape (64x64 < deep 128-wide tiles < 32x32).  The 64x64 kernel
+// (LDS-direct staging, dead fragments skipped, two independent workgroups per
+// CU) beats the register-staged deep kernels at equal padding (RI-K: 15 % on
+// the transform's 176 x 5263 blocks, 10 % on the K contraction's ~105-wide
+// ones); the deep kernels remain for C blocks of at least 128 on both sides
+// where their wider tile pads less.
 static int pick_cfg(int S, int kind, const sa_bsm* C, int tile) {
   if (tile >= 0 && tile < kNumOzCfgs && kOzCfgs[tile].S == S && kOzCfgs[tile].kind == kind) return tile;
   auto mean = [](const std::vector<int32_t>& v) {
@@ -818,7 +871,7 @@ static int pick_cfg(int S, int kind, const sa_bsm* C, int tile) {
     const OzCfg& k = kOzCfgs[i];
     if (k.S != S || k.kind != kind) continue;
     if (k.deep && !large) continue;
-    const double w = k.deep ? 0.7 : (k.fm == 2 ? 1.0 : 1.6);
+    const double w = k.deep ? 1.15 : (k.fm == 2 ? 1.0 : 1.6);
     const double cost = w * padded_ratio(C->rsz, C->csz, 32 * k.fm, 32 * k.fn);
     if (cost < best_cost) {
       best_cost = cost;
