@@ -412,14 +412,17 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 #pragma unroll
         for (int w = 0; w < S; ++w) s32[w][i][j] = acc4{0, 0, 0, 0};
       }
-    int er[EA][4], ec[EB];
-    auto load_exps = [&](const OzProd& pr) {
+    // exponents of the current product (er, ec) and of the next one (ern,
+    // ecn: loaded with the next product's first stage, so the load is in
+    // flight during a stage of MFMAs instead of in front of a barrier)
+    int er[EA][4], ec[EB], ern[EA][4], ecn[EB];
+    auto load_exps = [&](const OzProd& pr, int(&r)[EA][4], int(&cc)[EB]) {
 #pragma unroll
       for (int i = 0; i < NA; ++i)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) er[i][v] = gload1(pr.ea + (rb0 + wm * FM + i) * 16 + 4 * lg + v);
+        for (int v = 0; v < 4; ++v) r[i][v] = gload1(pr.ea + (rb0 + wm * FM + i) * 16 + 4 * lg + v);
 #pragma unroll
-      for (int j = 0; j < NB; ++j) ec[j] = gload1(pr.eb + (cb0 + wn * FN + j) * 16 + li);
+      for (int j = 0; j < NB; ++j) cc[j] = gload1(pr.eb + (cb0 + wn * FN + j) * 16 + li);
     };
     auto compute = [&](auto par) {
       const uint4(*As)[64] = decltype(par)::value ? As1 : As0;
@@ -470,7 +473,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
       // record of the following product, loaded one product ahead
       OzProd pf = p + 1 < o.p1 ? gload1(prods + p + 1) : pr;
       int ks = 0;
-      load_exps(pr);
+      load_exps(pr, er, ec);
       gload(pr, 0, I0{});
       __syncthreads();
       // one stage computing from buffer P while the next loads into P ^ 1;
@@ -488,6 +491,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
           if (np != p) {
             pn = pf;
             if (np + 1 < o.p1) pf = gload1(prods + np + 1);
+            load_exps(pn, ern, ecn);
           }
           gload(pn, nks, std::integral_constant<int, P ^ 1>{});
         }
@@ -495,7 +499,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
         const int done = ks + 1;
         if (np != p || (done & (FULL - 1)) == 0) fold();
         if (!has_next) return true;
-        if (np != p) load_exps(pn);
+        if (np != p) {
+#pragma unroll
+          for (int i = 0; i < NA; ++i)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) er[i][v] = ern[i][v];
+#pragma unroll
+          for (int j = 0; j < NB; ++j) ec[j] = ecn[j];
+        }
         __syncthreads();
         p = np;
         ks = nks;
