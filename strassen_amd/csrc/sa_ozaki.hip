@@ -348,6 +348,26 @@ __device__ inline void oz_dispatch(int na, int nb, F&& f) {
 // wave's live fragment counts, so each specialisation issues its MFMAs as one
 // straight-line block (a branch per fragment would put every MFMA in its own
 // basic block and serialise it behind the LDS reads).
+// Issue order of one stage's LDS reads and MFMAs: the B slices and the first
+// A slice up front, then each A slice's MFMAs with the next A slice read right
+// after the first of them, so its LDS latency hides behind the rest.
+template <int S, int NA, int NB, int PA>
+__device__ inline void oz_sched_slice() {
+  constexpr int n = (S - PA) * NA * NB;
+  if constexpr (PA + 1 < S) {
+    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);      // MFMA
+    __builtin_amdgcn_sched_group_barrier(0x100, NA, 0);     // LDS read
+    __builtin_amdgcn_sched_group_barrier(0x008, n - 1, 0);  // MFMA
+  } else {
+    __builtin_amdgcn_sched_group_barrier(0x008, n, 0);
+  }
+}
+template <int S, int NA, int NB, int... PA>
+__device__ inline void oz_sched(std::integer_sequence<int, PA...>) {
+  __builtin_amdgcn_sched_group_barrier(0x100, S * NB + NA, 0);
+  (oz_sched_slice<S, NA, NB, PA>(), ...);
+}
+
 template <int S, int KIND, int FM, int FN>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
     k_gemm_ozaki(const GemmOut* __restrict__ outs, const OzProd* __restrict__ prods,
@@ -432,19 +452,23 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
       for (int q = 0; q < S; ++q)
 #pragma unroll
         for (int j = 0; j < NB; ++j) bq[q][j] = Bs[q * RB + wn * FN + j][lane];
+      uint4 aq[S][EA];
+#pragma unroll
+      for (int i = 0; i < NA; ++i) aq[0][i] = As[wm * FM + i][lane];
 #pragma unroll
       for (int pa = 0; pa < S; ++pa) {
-        uint4 aq[EA];
+        if (pa + 1 < S)
 #pragma unroll
-        for (int i = 0; i < NA; ++i) aq[i] = As[pa * RA + wm * FM + i][lane];
+          for (int i = 0; i < NA; ++i) aq[pa + 1][i] = As[(pa + 1) * RA + wm * FM + i][lane];
 #pragma unroll
         for (int pb = 0; pb < S - pa; ++pb)
 #pragma unroll
           for (int i = 0; i < NA; ++i)
 #pragma unroll
             for (int j = 0; j < NB; ++j)
-              s32[pa + pb][i][j] = oz_mfma(std::integral_constant<int, KIND>{}, aq[i], bq[pb][j], s32[pa + pb][i][j]);
+              s32[pa + pb][i][j] = oz_mfma(std::integral_constant<int, KIND>{}, aq[pa][i], bq[pb][j], s32[pa + pb][i][j]);
       }
+      if constexpr (NA > 0 && NB > 0) oz_sched<S, NA, NB>(std::make_integer_sequence<int, S>{});
     };
     // acc += sum_w s_w 2^(e_r + f_c + 2 TS - beta (w + 2)), s = 0: t = sum_w
     // s_w 2^(-beta w) by Horner steps, one scaling per element
