@@ -68,6 +68,25 @@ def timed(ctx, fn, steps):
 
 
 # ----------------------------------------------------------------- RI-K --
+RIK_PROFILE = "profiles/gemm_ozaki_int8_rik_config4.md"
+
+
+def profiled_traffic(path):
+    """Bytes below L2 per launch of the dominant kernel, as measured by
+    rocprofv3 (FETCH_SIZE and WRITE_SIZE, scaled by a copy-kernel calibration)
+    for this exact workload and recorded in the committed profile summary;
+    null for any other workload."""
+    if path:
+        try:
+            with open(os.path.join(ROOT, path)) as fh:
+                for line in fh:
+                    if line.startswith("traffic_bytes_per_launch:"):
+                        return {"traffic": int(line.split(":")[1]), "traffic_source": path}
+        except OSError:
+            pass
+    return {"traffic": None}
+
+
 def cpu_baseline_rik(w, X, budget_s=15.0):
     """Oracle transform on a bounded sample: P-groups 0..g-1 against all Q for
     the first column block of B, grown until about budget_s of work."""
@@ -203,7 +222,8 @@ def bench_rik(args):
                            "newton_schulz_iterations": ns_it, "newton_schulz_residual": ns_res}},
         "roofline": {"bound": "mfma", "achieved": H["lowp_tops"], "peak": H["lowp_peak"],
                      "unit": "TOP/s" if H["slice_type"] == "int8" else "TFLOP/s",
-                     "frac": H["lowp_frac"], "traffic": None, "kernel": "k_gemm_ozaki",
+                     "frac": H["lowp_frac"], "kernel": "k_gemm_ozaki",
+                     **profiled_traffic(RIK_PROFILE if (args.n_occ == 200 and head == "int8_5_5") else None),
                      "kernel_avg_ms": H["kernel_avg_ms"],
                      "note": "achieved = executed low-precision slice-product operations (pairs x algorithmic "
                              "flop) / k_gemm_ozaki time over the timed steps; the FP64 path's k_gemm_fp64 is in "
