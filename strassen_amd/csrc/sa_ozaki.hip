@@ -337,8 +337,8 @@ __device__ inline void oz_dispatch(int na, int nb, F&& f) {
   else f(C0{}, C0{});
 }
 
-// One workgroup of 4 waves (2 x 2) owns a (32 FM) x (32 FN) output tile; each
-// wave owns (16 FM) x (16 FN).  A stage is one MFMA k-step for all S slices of
+// One workgroup of WM x WN waves (default 2 x 2) owns a (16 FM WM) x
+// (16 FN WN) output tile; each wave owns (16 FM) x (16 FN).  A stage is one MFMA k-step for all S slices of
 // the tile's op(A) rows and op(B) columns, i.e. S (2 FM + 2 FN) fragments of
 // 1 KB, double-buffered in LDS and loaded straight from global memory into
 // LDS (no register staging).  The stream of stages runs over the products of
@@ -368,15 +368,16 @@ __device__ inline void oz_sched(std::integer_sequence<int, PA...>) {
   (oz_sched_slice<S, NA, NB, PA>(), ...);
 }
 
-template <int S, int KIND, int FM, int FN>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+template <int S, int KIND, int FM, int FN, int WM = 2, int WN = 2>
+__global__ void __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_eu(2)))
     k_gemm_ozaki(const GemmOut* __restrict__ outs, const OzProd* __restrict__ prods,
                  const GemmItem* __restrict__ items, int64_t nitems, double alpha) {
   constexpr int BETA = oz_beta(KIND), TS = oz_tshift(KIND);
   typedef typename OzAcc<KIND>::type acc4;
-  constexpr int RA = 2 * FM, RB = 2 * FN;    // 16-blocks of rows / columns per tile
-  constexpr int NCA = S * RA, NCB = S * RB;  // fragments per stage
-  constexpr int LA = (NCA + 3) / 4, LB = (NCB + 3) / 4;
+  constexpr int NW = WM * WN;                  // waves
+  constexpr int RA = WM * FM, RB = WN * FN;    // 16-blocks of rows / columns per tile
+  constexpr int NCA = S * RA, NCB = S * RB;    // fragments per stage
+  constexpr int LA = (NCA + NW - 1) / NW, LB = (NCB + NW - 1) / NW;
   constexpr int FULL = oz_full_fold(KIND);
   // the two stage buffers are separate objects: the compiler then sees that
   // the LDS reads of one stage do not wait for the loads into the other
@@ -385,7 +386,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 
   const int tid = threadIdx.x, lane = tid & 63;
   const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
-  const int wm = wid & 1, wn = wid >> 1;
+  const int wm = wid % WM, wn = wid / WM;
   const int li = lane & 15, lg = lane >> 4;
   const int64_t it = xcd_remap(blockIdx.x, nitems);
   const GemmItem item = items ? gload1(items + it) : GemmItem{(int32_t)it, 0, 0};
@@ -406,13 +407,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
     const int64_t psa = (int64_t)pr.nvba * pr.nks * 64, psb = (int64_t)pr.nvbb * pr.nks * 64;
 #pragma unroll
     for (int r = 0; r < LA; ++r) {
-      const int c = wid + 4 * r;
+      const int c = wid + NW * r;
       const int p = c / RA, vb = rb0 + c % RA;
       if (c < NCA && vb < nrb) lds_dma16(a4 + p * psa + ((int64_t)vb * pr.nks + ks) * 64 + lane, &As[c][0]);
     }
 #pragma unroll
     for (int r = 0; r < LB; ++r) {
-      const int c = wid + 4 * r;
+      const int c = wid + NW * r;
       const int p = c / RB, vb = cb0 + c % RB;
       if (c < NCB && vb < ncb) lds_dma16(b4 + p * psb + ((int64_t)vb * pr.nks + ks) * 64 + lane, &Bs[c][0]);
     }
@@ -783,15 +784,18 @@ using OzFn = void (*)(const GemmOut*, const OzProd*, const GemmItem*, int64_t, d
 struct OzCfg {
   int S;
   int kind;
-  int fm, fn;
-  bool deep;
+  int fm, fn;    // workgroup tile (32 fm) x (32 fn)
+  bool large;    // only for C blocks of at least 128 on both sides (one workgroup per CU)
+  double w;      // relative cost per useful flop, for the choice below
   int threads;
   OzFn fn_;
 };
 
-#define SA_OZ(S, K, FM, FN) OzCfg{S, K, FM, FN, false, 256, k_gemm_ozaki<S, K, FM, FN>}
+#define SA_OZ(S, K, FM, FN) OzCfg{S, K, FM, FN, false, FM == 2 ? 1.0 : 1.6, 256, k_gemm_ozaki<S, K, FM, FN>}
+// wide: (RM x RN) waves of 32 x 32 with LDS-direct staging, tile (32 RM) x (32 RN)
+#define SA_OZW(S, K, RM, RN, W) OzCfg{S, K, RM, RN, true, W, 64 * RM * RN, k_gemm_ozaki<S, K, 2, 2, RM, RN>}
 // deep: (WM x WN) = (RM x RN) waves of 32 x 32, workgroup tile (32 RM) x (32 RN)
-#define SA_OZD(S, K, RM, RN) OzCfg{S, K, RM, RN, true, 64 * RM * RN, k_gemm_ozaki_deep<S, K, RM, RN, 2, 2>}
+#define SA_OZD(S, K, RM, RN) OzCfg{S, K, RM, RN, true, 1.15, 64 * RM * RN, k_gemm_ozaki_deep<S, K, RM, RN, 2, 2>}
 
 static const OzCfg kOzCfgs[] = {
     SA_OZ(3, 0, 2, 2),  SA_OZ(4, 0, 2, 2),  SA_OZ(5, 0, 2, 2),   // 0-2: FP16 64x64
@@ -806,6 +810,8 @@ static const OzCfg kOzCfgs[] = {
     SA_OZ(4, 2, 1, 1),  SA_OZ(5, 2, 1, 1),  SA_OZ(6, 2, 1, 1),   // 25-27: INT8 32x32
     SA_OZD(4, 2, 4, 2), SA_OZD(5, 2, 4, 2),                      // 28-29: INT8 128x64, deep
     SA_OZD(4, 2, 2, 4), SA_OZD(5, 2, 2, 4),                      // 30-31: INT8 64x128, deep
+    SA_OZW(5, 2, 2, 4, 1.2), SA_OZW(5, 2, 4, 2, 1.2),            // 32-33: INT8 64x128 / 128x64, wide
+    SA_OZW(4, 2, 2, 4, 1.2), SA_OZW(4, 2, 4, 2, 1.2),            // 34-35: INT8 S4 64x128 / 128x64, wide
 };
 
 __global__ void k_count_tiles_oz(const GemmOut* __restrict__ outs, int64_t nouts, int TM, int TN,
@@ -905,9 +911,8 @@ static int pick_cfg(int S, int kind, const sa_bsm* C, int tile) {
   for (int i = 0; i < kNumOzCfgs; ++i) {
     const OzCfg& k = kOzCfgs[i];
     if (k.S != S || k.kind != kind) continue;
-    if (k.deep && !large) continue;
-    const double w = k.deep ? 1.15 : (k.fm == 2 ? 1.0 : 1.6);
-    const double cost = w * padded_ratio(C->rsz, C->csz, 32 * k.fm, 32 * k.fn);
+    if (k.large && !large) continue;
+    const double cost = k.w * padded_ratio(C->rsz, C->csz, 32 * k.fm, 32 * k.fn);
     if (cost < best_cost) {
       best_cost = cost;
       best = i;

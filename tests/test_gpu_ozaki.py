@@ -88,14 +88,16 @@ DEEP = [(12, 3, sa.SLICE_FP16), (13, 4, sa.SLICE_FP16), (14, 5, sa.SLICE_FP16), 
         (16, 4, sa.SLICE_FP16), (17, 5, sa.SLICE_FP16), (18, 4, sa.SLICE_BF16), (19, 5, sa.SLICE_BF16),
         (20, 4, sa.SLICE_BF16), (21, 5, sa.SLICE_BF16), (22, 4, sa.SLICE_INT8), (23, 5, sa.SLICE_INT8),
         (24, 6, sa.SLICE_INT8), (25, 4, sa.SLICE_INT8), (26, 5, sa.SLICE_INT8), (28, 4, sa.SLICE_INT8),
-        (29, 5, sa.SLICE_INT8), (30, 4, sa.SLICE_INT8), (31, 5, sa.SLICE_INT8)]
+        (29, 5, sa.SLICE_INT8), (30, 4, sa.SLICE_INT8), (31, 5, sa.SLICE_INT8), (32, 5, sa.SLICE_INT8),
+        (33, 5, sa.SLICE_INT8), (34, 4, sa.SLICE_INT8), (35, 4, sa.SLICE_INT8)]
 
 
 @pytest.mark.parametrize("tile,slices,stype", DEEP)
 @pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0)])
 def test_deep_pipelined_kernels(tile, slices, stype, ta, tb):
-    """128x64 / 64x128 deep-pipelined kernels on ragged 150-blocks (several
-    tiles, partial fragments, multi-product outputs and k-steps of 32)."""
+    """Every kernel table entry by id (64x64, 32x32, 128x64 / 64x128 deep and
+    wide) on ragged 150-blocks (several tiles, partial fragments and waves
+    without live fragments, multi-product outputs, several k-steps)."""
     bs = 150
     rng = np.random.default_rng(500 + tile + ta)
     ni, nk, nj = 2 * bs + 7, 3 * bs + 3, 2 * bs - 1
