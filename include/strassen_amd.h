@@ -198,6 +198,22 @@ int sa_inv_sqrt(sa_ctx_t ctx, int64_t n, const double* M, double* X, sa_bsm_t X_
 int sa_ri_k_step(sa_bsm_t X, sa_bsm_t B, sa_bsm_t Bp, sa_bsm_t K, int32_t n_mu, const int32_t* mu_sizes,
                  int32_t n_ic, const int32_t* i_chunks, double thresh, const sa_opts_t* opts_transform,
                  const sa_opts_t* opts_k);
+/* Occupied RI-K exchange step, sparse path of occ_ri_k_step.cpp:83-131
+ * (one team: K_mu_i_sub accumulated over all P, then K += K_sub), entirely on
+ * the device: K_{mu j} = sum_{P,i} (mu i|P)(ij|P).  ints3c is the compressed
+ * 3-centre tensor (n_vec2 x n_aux, column-major: column P holds the pairs of
+ * aux function P), keys[n] = r + n_bf c places pair n in the n_bf x n_bf
+ * matrix (v2m_key format, occ_ri_k_step.cpp:133-146; a key named twice keeps
+ * its later value), mo the n_bf x n_occ occupied coefficients, K the n_bf x
+ * n_occ result (all host, column-major).  The integrals are uploaded once and
+ * decompressed per P on the device straight into the block arena of U_P (no
+ * dense n_bf x n_bf matrix), with U's copy-in drop rule at 1e-20 as in the
+ * reference; then mui = U C (norms refreshed), ij = C^T mui, ijs = ij + ij^T,
+ * Ks += mui ijs, each screened at thresh and run in the mode of opts (NULL:
+ * FP64).  Errors: SA_ERR_INVALID for a key beyond n_bf^2 (the reference's
+ * assert, occ_ri_k_step.cpp:139) or bad sizes. */
+int sa_occ_ri_k(sa_ctx_t ctx, int64_t n_bf, int64_t n_aux, int64_t n_occ, int64_t n_vec2, const double* ints3c,
+                const double* mo, const uint64_t* keys, int32_t bs, double thresh, const sa_opts_t* opts, double* K);
 
 /* ------------------------------------------------------ hybrid Strassen -- */
 /* matmult_strassen_dryrun(A, B, thresh_per_block) -- strassen.hpp:510-679.

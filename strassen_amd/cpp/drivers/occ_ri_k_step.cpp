@@ -1,11 +1,14 @@
 // occ_ri_k_step -- argv-compatible driver of the reference's occ_ri_k_step.cpp:14-131
 //   <N_bf> <N_aux> <N_occ> <N_vec2> <3c file> <MO file> <v2m_key file> <blocksize> <threshold>
 // Dense and block-sparse occupied RI-K, K_{mu j} = sum_{P,i} (mu i|P)(ij|P), both
-// on the device: the dense path through the device DGEMM, the sparse path
-// through the drop-in BlockSparseMatrix (same call sequence per aux index P).
+// on the device: the dense path through the device DGEMM per aux index P, the
+// sparse path as one sa_occ_ri_k call (device-side decompression straight into
+// block form, then the reference's screened call sequence per P).
 #include <chrono>
+#include <cstdint>
 #include <cstdio>
 #include <string>
+#include <vector>
 
 #include "blocksparsematrix.h"
 #include "matrix.h"
@@ -66,34 +69,23 @@ int main(int argc, char** argv) {
   std::printf("L2 norm of output = %.15e\n", l2);
 
   std::printf("--- sparse occ-RIK  ---\n");
-  BlockSparseMatrix<double> K_bsm(N_bf, N_occ, bs, bs, 0.0);
-  BlockSparseMatrix<double> C_bsm(C, bs, bs, 0.0);
+  // The P loop of occ_ri_k_step.cpp:89-118 as one device call: the compressed
+  // integrals are uploaded once and decompressed per P on the device straight
+  // into block form (no dense N_bf x N_bf matrix, no per-P host traffic).
+  std::vector<uint64_t> key64(N_vec2);
+  for (size_t n = 0; n < N_vec2; ++n) key64[n] = (uint64_t)keys.elem(n, 0);
+  Matrix<double> Ksp(0.0, N_bf, N_occ);
   for (int it = 0; it < 3; ++it) {
-    K_bsm.fill_with_values(0.0);
     const auto t0 = std::chrono::steady_clock::now();
-    Matrix<double> dec(0.0, N_bf, N_bf);
-    BlockSparseMatrix<double> U(N_bf, N_bf, bs, bs, 1e-20);
-    BlockSparseMatrix<double> mui(N_bf, N_occ, bs, bs, 0.0);
-    BlockSparseMatrix<double> ij(N_occ, N_occ, bs, bs, 0.0);
-    BlockSparseMatrix<double> ijs(ij);
-    BlockSparseMatrix<double> Ks(N_bf, N_occ, bs, bs, 0.0);
-    Ks.fill_with_values(0.0);
-    for (size_t P = 0; P < N_aux; ++P) {
-      decompress_integrals(dec, P, ints, keys);
-      U.copy_from_input_matrix(dec);
-      matmult(mui, U, false, C_bsm, false, thresh, 1.0, 0.0);
-      mui.calc_frobenius_norms();
-      matmult(ij, C_bsm, true, mui, false, thresh, 1.0, 0.0);
-      ijs = ij;
-      ijs.add_transpose(ij);
-      matmult(Ks, mui, false, ijs, false, thresh, 1.0, 1.0);
-    }
-    K_bsm += Ks;
+    sa_cpp::check(sa_occ_ri_k(sa_cpp::context(), (int64_t)N_bf, (int64_t)N_aux, (int64_t)N_occ, (int64_t)N_vec2,
+                              ints.data_ptr(), C.data_ptr(), key64.data(), (int32_t)bs, thresh, nullptr,
+                              Ksp.data_ptr()),
+                  "sa_occ_ri_k");
     const double us = (double)std::chrono::duration_cast<std::chrono::microseconds>(std::chrono::steady_clock::now() - t0).count();
     std::printf("  [%d] %.4f s  (%.4f GFLOPs)\n", it + 1, 1e-6 * us, 1e-3 * total_flops / us);
-    auto diff = K_bsm.to_matrix() - K;
+    auto diff = Ksp - K;
     std::printf("relative RMSD = %e\n", diff.calc_frobenius_norm() / l2);
   }
-  std::printf("L2 norm of sparse output = %.15e\n", K_bsm.to_matrix().calc_frobenius_norm());
+  std::printf("L2 norm of sparse output = %.15e\n", Ksp.calc_frobenius_norm());
   return 0;
 }

@@ -15,19 +15,6 @@
 namespace sa {
 
 // ------------------------------------------------------------- primitives --
-// Canonical block norm: lane l accumulates fma(x,x,acc) over e = l, l+64, ...;
-// a halving tree s_l += s_{l+off} (off = 32..1); IEEE sqrt in lane 0.
-__device__ inline double wave_block_norm(const double* __restrict__ x, int64_t n, int lane) {
-  double a = 0.0;
-  for (int64_t e = lane; e < n; e += 64) a = fma(x[e], x[e], a);
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) {
-    double y = __shfl_down(a, off, 64);
-    a = a + y;
-  }
-  return sqrt(a);
-}
-
 __global__ void k_fill_i64(int64_t* __restrict__ p, int64_t n, int64_t v) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;

@@ -89,6 +89,19 @@ __device__ inline int64_t xcd_remap(int64_t bid, int64_t n) {
   return x * q + (x < r ? x : r) + t;
 }
 
+// Canonical block norm: lane l accumulates fma(x,x,acc) over e = l, l+64, ...;
+// a halving tree s_l += s_{l+off} (off = 32..1); IEEE sqrt in lane 0.
+__device__ inline double wave_block_norm(const double* __restrict__ x, int64_t n, int lane) {
+  double a = 0.0;
+  for (int64_t e = lane; e < n; e += 64) a = fma(x[e], x[e], a);
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    double y = __shfl_down(a, off, 64);
+    a = a + y;
+  }
+  return sqrt(a);
+}
+
 // Load through the global address space.  Pointers reached through records
 // are generic, so plain dereferences compile to flat loads, which also count
 // on the LDS wait counter: every wait for an LDS read would then wait for the

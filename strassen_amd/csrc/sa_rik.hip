@@ -11,6 +11,7 @@
 //    tensormult form 2), cached norms of B' refreshed, then
 //    K = sum_{P,i} B'(P,mu,i) B'(P,nu,i) (tensormult form 1) as one transposed
 //    screened multiply over a reshape view of B' whose rows are (P, i) pairs.
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -135,6 +136,51 @@ static void inv_sqrt_device(sa_context* c, int64_t n, const double* dM, double* 
   SA_CHECK_HIP(hipGetLastError());
 }
 
+// ------------------------------------------------------------ occ-RI-K --
+// Device-side decompression of the compressed 3-centre integrals
+// (occ_ri_k_step.cpp:133-146) straight into the block arena of U_P: every key
+// (r + n_bf c of the n_bf x n_bf matrix) is resolved once on the host to its
+// position in a fixed slot layout of the blocks any key touches; per aux
+// function P the N_vec2 values are scattered there (positions no key names stay
+// zero for every P, as in the reference's reused dense buffer), then each
+// touched block gets the canonical norm and the copy-in drop rule
+// (blocksparsematrix.hpp:55-59) of ints_3c_decompressed_bsm.copy_from_input_matrix.
+__global__ void k_occ_scatter(const double* __restrict__ vals, const int64_t* __restrict__ dest, int64_t n,
+                              double* __restrict__ arena) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t d = dest[i];
+  if (d >= 0) arena[d] = vals[i];
+}
+
+__global__ void __launch_bounds__(256) k_occ_block_norms(const double* __restrict__ arena,
+                                                         const int64_t* __restrict__ slot,
+                                                         const int32_t* __restrict__ rsz,
+                                                         const int32_t* __restrict__ csz,
+                                                         const int32_t* __restrict__ rscr,
+                                                         const int32_t* __restrict__ cscr, int32_t nrb, int64_t nb,
+                                                         double thresh, int64_t* __restrict__ off,
+                                                         double* __restrict__ norm) {
+  const int lane = threadIdx.x & 63;
+  const int64_t b = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (b >= nb) return;
+  const int64_t sl = slot[b];
+  if (sl < 0) {  // all-zero block: dropped by the copy-in rule (threshold > 0)
+    if (lane == 0) {
+      off[b] = -1;
+      norm[b] = 0.0;
+    }
+    return;
+  }
+  const int rb = (int)(b % nrb), cb = (int)(b / nrb);
+  const double est = wave_block_norm(arena + sl, (int64_t)rsz[rb] * csz[cb], lane);
+  if (lane == 0) {
+    const bool drop = est < thresh * (double)((int64_t)cscr[cb] * (int64_t)rscr[rb]);
+    off[b] = drop ? -1 : sl;
+    norm[b] = drop ? 0.0 : est;
+  }
+}
+
 }  // namespace sa
 
 using namespace sa;
@@ -224,6 +270,105 @@ int sa_ri_k_step(sa_bsm_t X, sa_bsm_t B, sa_bsm_t Bp, sa_bsm_t K, int32_t n_mu, 
     }
     sa_bsm_destroy(view);
   });
+}
+
+int sa_occ_ri_k(sa_ctx_t c, int64_t n_bf, int64_t n_aux, int64_t n_occ, int64_t n_vec2, const double* ints3c,
+                const double* mo, const uint64_t* keys, int32_t bs, double thresh, const sa_opts_t* opts, double* K) {
+  std::vector<sa_bsm_t> mats;
+  const int rc = guard([&] {
+    SA_REQUIRE(c && mo && K && (n_vec2 == 0 || (ints3c && keys)), "null argument");
+    SA_REQUIRE(n_bf > 0 && n_occ > 0 && n_aux >= 0 && n_vec2 >= 0 && bs > 0, "occ_ri_k: bad sizes");
+    const uint64_t nn = (uint64_t)n_bf * (uint64_t)n_bf;
+    for (int64_t n = 0; n < n_vec2; ++n)  // occ_ri_k_step.cpp:139
+      SA_REQUIRE(keys[n] < nn, "occ_ri_k: key %lld = %llu is beyond the %lld x %lld matrix", (long long)n,
+                 (unsigned long long)keys[n], (long long)n_bf, (long long)n_bf);
+    SA_CHECK_HIP(hipSetDevice(c->device));
+    auto make = [&](int64_t r, int64_t q, double t) {
+      sa_bsm_t m = nullptr;
+      const int e = sa_bsm_create(c, r, q, bs, bs, t, &m);
+      if (e != SA_OK) throw Error{e};
+      mats.push_back(m);
+      return m;
+    };
+    auto ok = [](int e) {
+      if (e != SA_OK) throw Error{e};
+    };
+    // occ_ri_k_step.cpp:84-98 (U's copy-in threshold is the reference's 1e-20)
+    sa_bsm_t Kb = make(n_bf, n_occ, 0.0), Cb = make(n_bf, n_occ, 0.0), U = make(n_bf, n_bf, 1e-20);
+    sa_bsm_t mui = make(n_bf, n_occ, 0.0), ij = make(n_occ, n_occ, 0.0), ijs = make(n_occ, n_occ, 0.0);
+    sa_bsm_t Ks = make(n_bf, n_occ, 0.0);
+    ok(sa_bsm_from_dense(Cb, mo, n_bf));
+    ok(sa_bsm_fill(Kb, 0.0));
+    ok(sa_bsm_fill(Ks, 0.0));
+
+    // slot layout of U: the blocks any key touches, column-major inside a block
+    const int64_t nrb = U->nrb, nb = U->nb();
+    std::vector<int64_t> slot((size_t)nb, -1), dest((size_t)n_vec2);
+    for (int64_t n = 0; n < n_vec2; ++n) {
+      const int64_t r = (int64_t)(keys[n] % (uint64_t)n_bf), q = (int64_t)(keys[n] / (uint64_t)n_bf);
+      slot[(size_t)(r / bs + (q / bs) * nrb)] = 0;
+    }
+    int64_t total = 0;
+    for (int64_t b = 0; b < nb; ++b)
+      if (slot[(size_t)b] >= 0) {
+        slot[(size_t)b] = total;
+        total += (int64_t)U->rsz[(size_t)(b % nrb)] * U->csz[(size_t)(b / nrb)];
+      }
+    for (int64_t n = 0; n < n_vec2; ++n) {
+      const int64_t r = (int64_t)(keys[n] % (uint64_t)n_bf), q = (int64_t)(keys[n] / (uint64_t)n_bf);
+      const int64_t rb = r / bs, cb = q / bs;
+      dest[(size_t)n] = slot[(size_t)(rb + cb * nrb)] + (r - U->roff[(size_t)rb]) +
+                        (q - U->coff[(size_t)cb]) * U->rsz[(size_t)rb];
+    }
+    // a key named twice: the reference's later write wins (one writer per position)
+    {
+      std::vector<int64_t> idx((size_t)n_vec2);
+      for (int64_t n = 0; n < n_vec2; ++n) idx[(size_t)n] = n;
+      std::stable_sort(idx.begin(), idx.end(), [&](int64_t a, int64_t b) { return keys[a] < keys[b]; });
+      for (size_t i = 0; i + 1 < idx.size(); ++i)
+        if (keys[idx[i]] == keys[idx[i + 1]]) dest[(size_t)idx[i]] = -1;
+    }
+    DevBuf& bints = c->buf("occ_ints");
+    DevBuf& bdest = c->buf("occ_dest");
+    DevBuf& bslot = c->buf("occ_slot");
+    bints.reserve(sizeof(double) * (size_t)std::max<int64_t>(1, n_vec2 * n_aux));
+    bdest.reserve(sizeof(int64_t) * (size_t)std::max<int64_t>(1, n_vec2));
+    bslot.reserve(sizeof(int64_t) * (size_t)std::max<int64_t>(1, nb));
+    if (n_vec2 * n_aux)
+      SA_CHECK_HIP(hipMemcpyAsync(bints.p, ints3c, sizeof(double) * (size_t)(n_vec2 * n_aux), hipMemcpyHostToDevice,
+                                  c->stream));
+    if (n_vec2)
+      SA_CHECK_HIP(hipMemcpyAsync(bdest.p, dest.data(), sizeof(int64_t) * (size_t)n_vec2, hipMemcpyHostToDevice,
+                                  c->stream));
+    SA_CHECK_HIP(hipMemcpyAsync(bslot.p, slot.data(), sizeof(int64_t) * (size_t)nb, hipMemcpyHostToDevice, c->stream));
+    bsm_ensure_capacity(U, total);
+    U->used = total;
+    fill_f64(c, U->data(), total, 0.0);
+    SA_CHECK_HIP(hipStreamSynchronize(c->stream));  // the host vectors above go out of scope
+
+    for (int64_t P = 0; P < n_aux; ++P) {  // occ_ri_k_step.cpp:100-118
+      if (n_vec2)
+        hipLaunchKernelGGL(k_occ_scatter, dim3((unsigned)ceil_div(n_vec2, 256)), dim3(256), 0, c->stream,
+                           (const double*)bints.as<double>() + P * n_vec2, (const int64_t*)bdest.as<int64_t>(),
+                           n_vec2, U->data());
+      hipLaunchKernelGGL(k_occ_block_norms, dim3((unsigned)ceil_div(nb, 4)), dim3(256), 0, c->stream,
+                         (const double*)U->data(), (const int64_t*)bslot.as<int64_t>(), (const int32_t*)U->d_rsz,
+                         (const int32_t*)U->d_csz, (const int32_t*)U->d_rscr, (const int32_t*)U->d_cscr, U->nrb, nb,
+                         U->thresh, U->d_off, U->d_norm);
+      SA_CHECK_HIP(hipGetLastError());
+      ok(sa_matmult(mui, U, 0, Cb, 0, thresh, 1.0, 0.0, opts));
+      ok(sa_bsm_calc_norms(mui));
+      ok(sa_matmult(ij, Cb, 1, mui, 0, thresh, 1.0, 0.0, opts));
+      ok(sa_bsm_copy(ijs, ij));
+      ok(sa_bsm_add_transpose(ijs, ij));
+      ok(sa_matmult(Ks, mui, 0, ijs, 0, thresh, 1.0, 1.0, opts));
+    }
+    ok(sa_bsm_add(Kb, Ks, 1.0));  // K_mu_i_bsm += K_mu_i_sub_bsm (occ_ri_k_step.cpp:122-125)
+    for (int64_t i = 0; i < n_bf * n_occ; ++i) K[i] = 0.0;
+    ok(sa_bsm_to_dense(Kb, K, n_bf));
+  });
+  for (sa_bsm_t m : mats) sa_bsm_destroy(m);
+  return rc;
 }
 
 }  // extern "C"

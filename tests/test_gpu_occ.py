@@ -1,0 +1,74 @@
+"""Device occupied RI-K (sa_occ_ri_k): the compressed 3-centre integrals are
+decompressed on the device straight into block form and the whole P loop of
+occ_ri_k_step.cpp:83-131 runs there.  Checked against the oracle's restatement
+of that loop (itself pinned by the h2o KAT, the closed form
+K = sum_P U_P C C^T (U_P + U_P^T) C and the built reference)."""
+import numpy as np
+import pytest
+
+import oracle
+import strassen_amd as sa
+from gpu_util import ctx, rel_fro
+
+pytestmark = pytest.mark.gpu
+
+H2O_NORM = 10.415071322740426  # ||K||_F of test_occ_ri_k_h2o.sh (24 113 5 322 PQm12 ... 3 1e-20)
+
+
+def test_h2o_kat(h2o):
+    K = sa.occ_ri_k(ctx(), 24, 113, 5, 322, h2o["pqm12"], h2o["mo"], h2o["keys"], 3, 1e-20)
+    Ko = oracle.occ_ri_k(24, 113, 5, 322, h2o["pqm12"], h2o["mo"], h2o["keys"], 3, 1e-20)
+    assert rel_fro(K, Ko) < 1e-14
+    assert abs(np.linalg.norm(K) - H2O_NORM) < 1e-12 * H2O_NORM
+
+
+def synthetic(seed, n_bf, n_aux, n_occ, frac=0.4, dup=0):
+    """Pairs (r <= c) of a random subset of the upper triangle with decaying
+    values, keys r + n_bf c as in the v2m_key format; dup extra keys repeat
+    earlier ones (a later value overwrites)."""
+    rng = np.random.default_rng(seed)
+    r, c = np.triu_indices(n_bf)
+    keep = rng.random(r.size) < frac
+    r, c = r[keep], c[keep]
+    keys = (r + n_bf * c).astype(np.uint64)
+    if dup:
+        keys = np.concatenate([keys, rng.choice(keys, dup)])
+    n_vec2 = keys.size
+    dist = np.abs(r - c).astype(float)
+    if dup:
+        dist = np.concatenate([dist, rng.random(dup) * n_bf])
+    ints = rng.standard_normal((n_vec2, n_aux)) * np.exp(-0.15 * dist)[:, None]
+    mo = rng.standard_normal((n_bf, n_occ))
+    return n_vec2, np.asfortranarray(ints), np.asfortranarray(mo), keys
+
+
+@pytest.mark.parametrize("n_bf,n_aux,n_occ,bs,thresh,dup",
+                         [(70, 40, 12, 8, 1e-10, 0), (45, 17, 9, 5, 1e-6, 0), (33, 12, 6, 4, 1e-10, 25)])
+def test_synthetic_vs_oracle(n_bf, n_aux, n_occ, bs, thresh, dup):
+    """Ragged last blocks, screening that drops products, repeated keys."""
+    n_vec2, ints, mo, keys = synthetic(n_bf + n_aux, n_bf, n_aux, n_occ, dup=dup)
+    K = sa.occ_ri_k(ctx(), n_bf, n_aux, n_occ, n_vec2, ints, mo, keys, bs, thresh)
+    Ko = oracle.occ_ri_k(n_bf, n_aux, n_occ, n_vec2, ints, mo, keys, bs, thresh)
+    assert np.linalg.norm(Ko) > 0
+    assert rel_fro(K, Ko) < 1e-12
+
+
+def test_split_precision(h2o):
+    """The three products in split precision (5 INT8 slices): bar 1e-10."""
+    n_vec2, ints, mo, keys = synthetic(7, 64, 30, 10)
+    o = sa.default_opts(mode=sa.MODE_OZAKI, slices=5, slice_type=sa.SLICE_INT8)
+    K = sa.occ_ri_k(ctx(), 64, 30, 10, n_vec2, ints, mo, keys, 8, 1e-10, opts=o)
+    Ko = oracle.occ_ri_k(64, 30, 10, n_vec2, ints, mo, keys, 8, 1e-10)
+    assert rel_fro(K, Ko) < 1e-10
+
+
+def test_key_out_of_range_is_an_error(h2o):
+    keys = h2o["keys"].copy()
+    keys[7] = 24 * 24
+    with pytest.raises(RuntimeError, match="beyond"):
+        sa.occ_ri_k(ctx(), 24, 113, 5, 322, h2o["pqm12"], h2o["mo"], keys, 3, 1e-20)
+
+
+def test_no_pairs():
+    K = sa.occ_ri_k(ctx(), 10, 4, 3, 0, np.zeros((0, 4)), np.ones((10, 3)), np.zeros(0, np.uint64), 4, 1e-10)
+    assert not K.any()
