@@ -245,6 +245,64 @@ def bench_rik(args):
 
 
 # ------------------------------------------------------ hybrid Strassen --
+def occ_problem(n_bf=600, n_aux=1200, n_occ=60, frac=0.35, seed=11):
+    """Synthetic occupied RI-K input in the reference's formats: pairs r <= c of
+    the basis kept with probability frac, values decaying with |r - c|, keys
+    r + n_bf c (v2m_key layout), MO coefficients N(0, 1)."""
+    rng = np.random.default_rng(seed)
+    r, c = np.triu_indices(n_bf)
+    keep = rng.random(r.size) < frac
+    r, c = r[keep], c[keep]
+    keys = (r + n_bf * c).astype(np.uint64)
+    ints = np.asfortranarray(rng.standard_normal((keys.size, n_aux)) * np.exp(-0.01 * np.abs(r - c))[:, None])
+    mo = np.asfortranarray(rng.standard_normal((n_bf, n_occ)))
+    return keys.size, ints, mo, keys
+
+
+def bench_occ(args):
+    """Device occupied RI-K (sa_occ_ri_k, occ_ri_k_step.cpp:83-131) on a synthetic
+    problem; the oracle's restatement timed on a bounded subset of aux functions
+    for the CPU baseline.  The call includes the upload of the compressed
+    integrals (host arrays in the reference's formats)."""
+    n_bf, n_aux, n_occ, bs, thresh = 600, 1200, 60, 12, 1e-10
+    n_vec2, ints, mo, keys = occ_problem(n_bf, n_aux, n_occ)
+    ctx = sa.Context(0)
+    run = lambda: sa.occ_ri_k(ctx, n_bf, n_aux, n_occ, n_vec2, ints, mo, keys, bs, thresh)
+    for _ in range(args.warmup):
+        K = run()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        K = run()
+    ms = (time.perf_counter() - t0) / args.steps * 1e3
+    fl = 2.0 * (n_bf * n_bf * n_occ * n_aux + 2.0 * n_bf * n_occ * n_occ * n_aux)  # occ_ri_k_step.cpp:75
+    rec = {"metric": METRIC, "value": round(fl / (ms * 1e-3) / 1e12, 4), "unit": "dense-equivalent TFLOP/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic (seeded compressed pairs and MO coefficients, host arrays)",
+           "config": {"workload": "occ_ri_k_device", "n_bf": n_bf, "n_aux": n_aux, "n_occ": n_occ, "n_vec2": n_vec2,
+                      "bs": bs, "thresh": thresh, "parallelism": "single GPU",
+                      "timing": "wall clock of the call, including the upload of the integrals"}}
+    if not args.no_cpu:
+        import oracle
+
+        g = 8
+        while True:
+            t0 = time.perf_counter()
+            oracle.occ_ri_k(n_bf, g, n_occ, n_vec2, np.asfortranarray(ints[:, :g]), mo, keys, bs, thresh)
+            dt = time.perf_counter() - t0
+            if dt > 3.0 or g >= n_aux:
+                break
+            g = min(n_aux, g * 4)
+        K_dev = sa.occ_ri_k(ctx, n_bf, g, n_occ, n_vec2, np.asfortranarray(ints[:, :g]), mo, keys, bs, thresh)
+        K_or = oracle.occ_ri_k(n_bf, g, n_occ, n_vec2, np.asfortranarray(ints[:, :g]), mo, keys, bs, thresh)
+        rate = fl * g / n_aux / dt
+        rec["cpu_baseline"] = {"value": round(rate / 1e12, 6), "unit": "dense-equivalent TFLOP/s", "cores": 1,
+                               "kind": "port", "sample": f"oracle P loop over the first {g} of {n_aux} aux functions "
+                                                         f"({dt:.1f} s), scaled by aux count",
+                               "rel_diff_device_vs_oracle_on_sample": rel(K_dev, K_or)}
+    print(json.dumps(rec))
+
+
 def bench_hybrid(args):
     """Large hybrid recursion: n=32768, 128x128 blocks, band |I-J|<=95 under
     exp(-0.05|I-J|), cutoff 2048 (Strassen depth <= 4).  One step = one
@@ -394,7 +452,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="rik", choices=["rik", "config1", "config2", "config3", "config5"])
+    ap.add_argument("--workload", default="rik", choices=["rik", "config1", "config2", "config3", "config5", "occ"])
     ap.add_argument("--tile", type=int, default=-1)
     ap.add_argument("--slices-k", type=int, default=4, help="split-precision slices (config3 workload)")
     ap.add_argument("--forms", default="int8:5:5,fp16:4:4",
@@ -411,6 +469,8 @@ def main():
         bench_rik(args)
     elif args.workload == "config5":
         bench_hybrid(args)
+    elif args.workload == "occ":
+        bench_occ(args)
     else:
         bench_gemm(args)
 
