@@ -267,7 +267,8 @@ def bench_occ(args):
     n_bf, n_aux, n_occ, bs, thresh = 600, 1200, 60, 12, 1e-10
     n_vec2, ints, mo, keys = occ_problem(n_bf, n_aux, n_occ)
     ctx = sa.Context(0)
-    run = lambda: sa.occ_ri_k(ctx, n_bf, n_aux, n_occ, n_vec2, ints, mo, keys, bs, thresh)
+    teams = 8
+    run = lambda: sa.occ_ri_k(ctx, n_bf, n_aux, n_occ, n_vec2, ints, mo, keys, bs, thresh, teams=teams)
     for _ in range(args.warmup):
         K = run()
     t0 = time.perf_counter()
@@ -280,7 +281,7 @@ def bench_occ(args):
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
            "data": "synthetic (seeded compressed pairs and MO coefficients, host arrays)",
            "config": {"workload": "occ_ri_k_device", "n_bf": n_bf, "n_aux": n_aux, "n_occ": n_occ, "n_vec2": n_vec2,
-                      "bs": bs, "thresh": thresh, "parallelism": "single GPU",
+                      "bs": bs, "thresh": thresh, "teams": teams, "parallelism": "single GPU, 8 streams",
                       "timing": "wall clock of the call, including the upload of the integrals"}}
     if not args.no_cpu:
         import oracle
@@ -293,7 +294,8 @@ def bench_occ(args):
             if dt > 3.0 or g >= n_aux:
                 break
             g = min(n_aux, g * 4)
-        K_dev = sa.occ_ri_k(ctx, n_bf, g, n_occ, n_vec2, np.asfortranarray(ints[:, :g]), mo, keys, bs, thresh)
+        K_dev = sa.occ_ri_k(ctx, n_bf, g, n_occ, n_vec2, np.asfortranarray(ints[:, :g]), mo, keys, bs, thresh,
+                            teams=1)
         K_or = oracle.occ_ri_k(n_bf, g, n_occ, n_vec2, np.asfortranarray(ints[:, :g]), mo, keys, bs, thresh)
         rate = fl * g / n_aux / dt
         rec["cpu_baseline"] = {"value": round(rate / 1e12, 6), "unit": "dense-equivalent TFLOP/s", "cores": 1,

@@ -198,9 +198,14 @@ int sa_inv_sqrt(sa_ctx_t ctx, int64_t n, const double* M, double* X, sa_bsm_t X_
 int sa_ri_k_step(sa_bsm_t X, sa_bsm_t B, sa_bsm_t Bp, sa_bsm_t K, int32_t n_mu, const int32_t* mu_sizes,
                  int32_t n_ic, const int32_t* i_chunks, double thresh, const sa_opts_t* opts_transform,
                  const sa_opts_t* opts_k);
-/* Occupied RI-K exchange step, sparse path of occ_ri_k_step.cpp:83-131
- * (one team: K_mu_i_sub accumulated over all P, then K += K_sub), entirely on
- * the device: K_{mu j} = sum_{P,i} (mu i|P)(ij|P).  ints3c is the compressed
+/* Occupied RI-K exchange step, sparse path of occ_ri_k_step.cpp:83-131,
+ * entirely on the device: K_{mu j} = sum_{P,i} (mu i|P)(ij|P).  The OpenMP
+ * team of the reference becomes `teams` members (<= 0: 8), each with its own
+ * stream and host thread: P is split as OpenMP's static schedule (contiguous
+ * ranges, the first n_aux % teams members one more), each member reuses its
+ * mui, ij, ijs and K_sub across its range, and the K_sub are added into K in
+ * member order (the reference adds them in a critical section in arrival
+ * order; teams = 1 is its one-thread run).  ints3c is the compressed
  * 3-centre tensor (n_vec2 x n_aux, column-major: column P holds the pairs of
  * aux function P), keys[n] = r + n_bf c places pair n in the n_bf x n_bf
  * matrix (v2m_key format, occ_ri_k_step.cpp:133-146; a key named twice keeps
@@ -213,7 +218,8 @@ int sa_ri_k_step(sa_bsm_t X, sa_bsm_t B, sa_bsm_t Bp, sa_bsm_t K, int32_t n_mu, 
  * FP64).  Errors: SA_ERR_INVALID for a key beyond n_bf^2 (the reference's
  * assert, occ_ri_k_step.cpp:139) or bad sizes. */
 int sa_occ_ri_k(sa_ctx_t ctx, int64_t n_bf, int64_t n_aux, int64_t n_occ, int64_t n_vec2, const double* ints3c,
-                const double* mo, const uint64_t* keys, int32_t bs, double thresh, const sa_opts_t* opts, double* K);
+                const double* mo, const uint64_t* keys, int32_t bs, double thresh, int32_t teams,
+                const sa_opts_t* opts, double* K);
 
 /* ------------------------------------------------------ hybrid Strassen -- */
 /* matmult_strassen_dryrun(A, B, thresh_per_block) -- strassen.hpp:510-679.

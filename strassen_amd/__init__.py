@@ -102,7 +102,7 @@ ABI = {
     "sa_daxpy": (INT, [P, I64, D, P, P]),
     "sa_inv_sqrt": (INT, [P, I64, P, P, P, I32, D, P, P]),
     "sa_ri_k_step": (INT, [P, P, P, P, I32, P, I32, P, D, P, P]),
-    "sa_occ_ri_k": (INT, [P, I64, I64, I64, I64, P, P, P, I32, D, P, P]),
+    "sa_occ_ri_k": (INT, [P, I64, I64, I64, I64, P, P, P, I32, D, I32, P, P]),
     "sa_strassen_dryrun": (INT, [P, P, D, P, I64, P, P, P]),
     "sa_matmult_strassen_sparse": (INT, [P, P, INT, P, INT, D, D, D, P]),
     "sa_strassen_last_stats": (INT, [P, P, P, P, P, P, P]),
@@ -515,15 +515,17 @@ def inv_sqrt(ctx: Context, M, X_bsm: BSM | None = None, max_iter=100, tol=1e-14,
     return X, it.value, res.value
 
 
-def occ_ri_k(ctx: Context, n_bf, n_aux, n_occ, n_vec2, ints3c, mo, keys, bs, thresh, opts: Opts | None = None):
+def occ_ri_k(ctx: Context, n_bf, n_aux, n_occ, n_vec2, ints3c, mo, keys, bs, thresh, teams=0,
+             opts: Opts | None = None):
     """Device occupied RI-K (sa_occ_ri_k): compressed integrals (n_vec2 x n_aux),
-    MO coefficients (n_bf x n_occ) and v2m keys in, K (n_bf x n_occ) out."""
+    MO coefficients (n_bf x n_occ) and v2m keys in, K (n_bf x n_occ) out;
+    teams = members of the P loop's team (0: 8)."""
     i3 = np.asfortranarray(ints3c, dtype=np.float64)
     m = np.asfortranarray(mo, dtype=np.float64)
     k = np.ascontiguousarray(np.ravel(keys), dtype=np.uint64)
     assert i3.shape == (n_vec2, n_aux) and m.shape == (n_bf, n_occ) and k.size == n_vec2
     K = np.zeros((n_bf, n_occ), order="F")
-    _check(lib().sa_occ_ri_k(ctx.h, n_bf, n_aux, n_occ, n_vec2, _ptr(i3), _ptr(m), _ptr(k), bs, thresh,
+    _check(lib().sa_occ_ri_k(ctx.h, n_bf, n_aux, n_occ, n_vec2, _ptr(i3), _ptr(m), _ptr(k), bs, thresh, teams,
                              None if opts is None else C.byref(opts), _ptr(K)))
     return K
 

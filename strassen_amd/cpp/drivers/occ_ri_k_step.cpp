@@ -78,7 +78,7 @@ int main(int argc, char** argv) {
   for (int it = 0; it < 3; ++it) {
     const auto t0 = std::chrono::steady_clock::now();
     sa_cpp::check(sa_occ_ri_k(sa_cpp::context(), (int64_t)N_bf, (int64_t)N_aux, (int64_t)N_occ, (int64_t)N_vec2,
-                              ints.data_ptr(), C.data_ptr(), key64.data(), (int32_t)bs, thresh, nullptr,
+                              ints.data_ptr(), C.data_ptr(), key64.data(), (int32_t)bs, thresh, 0, nullptr,
                               Ksp.data_ptr()),
                   "sa_occ_ri_k");
     const double us = (double)std::chrono::duration_cast<std::chrono::microseconds>(std::chrono::steady_clock::now() - t0).count();

@@ -16,6 +16,7 @@
 // land on the same XCD (hardware dispatches workgroup i to XCD i % 8).
 #include <algorithm>
 #include <map>
+#include <mutex>
 #include <tuple>
 
 #include "sa_internal.hpp"
@@ -452,17 +453,26 @@ __global__ void k_redirect_outs(const GemmOut* __restrict__ in, int64_t n, int64
   out[i] = o;
 }
 
+// shape class -> tile config; shared by all contexts (and the threads that
+// drive them), guarded by tune_mutex
 static std::map<std::tuple<int, int, int, int, int>, int>& tune_cache() {
   static std::map<std::tuple<int, int, int, int, int>, int> m;
   return m;
+}
+static std::mutex& tune_mutex() {
+  static std::mutex mu;
+  return mu;
 }
 
 static int round16(int x) { return (x + 15) / 16 * 16; }
 
 static int autotune(sa_context* c, const GemmBatch& b) {
   const auto key = std::make_tuple(b.ta, b.tb, b.max_m, b.max_n, b.max_k);
-  auto it = tune_cache().find(key);
-  if (it != tune_cache().end()) return it->second;
+  {
+    std::lock_guard<std::mutex> lk(tune_mutex());
+    auto it = tune_cache().find(key);
+    if (it != tune_cache().end()) return it->second;
+  }
   // sample size bounded by work: about 4e9 flops of products (at least 64 outputs)
   const double per_out = 2.0 * b.max_m * b.max_n * (double)b.max_k * ((double)b.nprods / (double)b.nouts);
   const int64_t S = std::max<int64_t>(1, std::min<int64_t>(
@@ -506,7 +516,10 @@ static int autotune(sa_context* c, const GemmBatch& b) {
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  tune_cache()[key] = best;
+  {
+    std::lock_guard<std::mutex> lk(tune_mutex());
+    tune_cache()[key] = best;
+  }
   return best;
 }
 

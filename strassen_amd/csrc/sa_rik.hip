@@ -13,6 +13,8 @@
 //    screened multiply over a reshape view of B' whose rows are (P, i) pairs.
 #include <algorithm>
 #include <cmath>
+#include <string>
+#include <thread>
 #include <vector>
 
 #include "sa_internal.hpp"
@@ -273,8 +275,10 @@ int sa_ri_k_step(sa_bsm_t X, sa_bsm_t B, sa_bsm_t Bp, sa_bsm_t K, int32_t n_mu, 
 }
 
 int sa_occ_ri_k(sa_ctx_t c, int64_t n_bf, int64_t n_aux, int64_t n_occ, int64_t n_vec2, const double* ints3c,
-                const double* mo, const uint64_t* keys, int32_t bs, double thresh, const sa_opts_t* opts, double* K) {
+                const double* mo, const uint64_t* keys, int32_t bs, double thresh, int32_t teams,
+                const sa_opts_t* opts, double* K) {
   std::vector<sa_bsm_t> mats;
+  std::vector<sa_ctx_t> ctxs;
   const int rc = guard([&] {
     SA_REQUIRE(c && mo && K && (n_vec2 == 0 || (ints3c && keys)), "null argument");
     SA_REQUIRE(n_bf > 0 && n_occ > 0 && n_aux >= 0 && n_vec2 >= 0 && bs > 0, "occ_ri_k: bad sizes");
@@ -283,26 +287,19 @@ int sa_occ_ri_k(sa_ctx_t c, int64_t n_bf, int64_t n_aux, int64_t n_occ, int64_t 
       SA_REQUIRE(keys[n] < nn, "occ_ri_k: key %lld = %llu is beyond the %lld x %lld matrix", (long long)n,
                  (unsigned long long)keys[n], (long long)n_bf, (long long)n_bf);
     SA_CHECK_HIP(hipSetDevice(c->device));
-    auto make = [&](int64_t r, int64_t q, double t) {
-      sa_bsm_t m = nullptr;
-      const int e = sa_bsm_create(c, r, q, bs, bs, t, &m);
-      if (e != SA_OK) throw Error{e};
-      mats.push_back(m);
-      return m;
-    };
     auto ok = [](int e) {
       if (e != SA_OK) throw Error{e};
     };
-    // occ_ri_k_step.cpp:84-98 (U's copy-in threshold is the reference's 1e-20)
-    sa_bsm_t Kb = make(n_bf, n_occ, 0.0), Cb = make(n_bf, n_occ, 0.0), U = make(n_bf, n_bf, 1e-20);
-    sa_bsm_t mui = make(n_bf, n_occ, 0.0), ij = make(n_occ, n_occ, 0.0), ijs = make(n_occ, n_occ, 0.0);
-    sa_bsm_t Ks = make(n_bf, n_occ, 0.0);
-    ok(sa_bsm_from_dense(Cb, mo, n_bf));
-    ok(sa_bsm_fill(Kb, 0.0));
-    ok(sa_bsm_fill(Ks, 0.0));
+    auto make = [&](sa_ctx_t x, int64_t r, int64_t q, double t) {
+      sa_bsm_t m = nullptr;
+      ok(sa_bsm_create(x, r, q, bs, bs, t, &m));
+      mats.push_back(m);
+      return m;
+    };
 
     // slot layout of U: the blocks any key touches, column-major inside a block
-    const int64_t nrb = U->nrb, nb = U->nb();
+    const int64_t nrb = ceil_div(n_bf, (int64_t)bs), nb = nrb * nrb;
+    auto bsize = [&](int64_t i) { return (int64_t)std::min<int64_t>(bs, n_bf - i * bs); };
     std::vector<int64_t> slot((size_t)nb, -1), dest((size_t)n_vec2);
     for (int64_t n = 0; n < n_vec2; ++n) {
       const int64_t r = (int64_t)(keys[n] % (uint64_t)n_bf), q = (int64_t)(keys[n] / (uint64_t)n_bf);
@@ -312,13 +309,12 @@ int sa_occ_ri_k(sa_ctx_t c, int64_t n_bf, int64_t n_aux, int64_t n_occ, int64_t 
     for (int64_t b = 0; b < nb; ++b)
       if (slot[(size_t)b] >= 0) {
         slot[(size_t)b] = total;
-        total += (int64_t)U->rsz[(size_t)(b % nrb)] * U->csz[(size_t)(b / nrb)];
+        total += bsize(b % nrb) * bsize(b / nrb);
       }
     for (int64_t n = 0; n < n_vec2; ++n) {
       const int64_t r = (int64_t)(keys[n] % (uint64_t)n_bf), q = (int64_t)(keys[n] / (uint64_t)n_bf);
       const int64_t rb = r / bs, cb = q / bs;
-      dest[(size_t)n] = slot[(size_t)(rb + cb * nrb)] + (r - U->roff[(size_t)rb]) +
-                        (q - U->coff[(size_t)cb]) * U->rsz[(size_t)rb];
+      dest[(size_t)n] = slot[(size_t)(rb + cb * nrb)] + (r - rb * bs) + (q - cb * bs) * bsize(rb);
     }
     // a key named twice: the reference's later write wins (one writer per position)
     {
@@ -328,46 +324,100 @@ int sa_occ_ri_k(sa_ctx_t c, int64_t n_bf, int64_t n_aux, int64_t n_occ, int64_t 
       for (size_t i = 0; i + 1 < idx.size(); ++i)
         if (keys[idx[i]] == keys[idx[i + 1]]) dest[(size_t)idx[i]] = -1;
     }
-    DevBuf& bints = c->buf("occ_ints");
-    DevBuf& bdest = c->buf("occ_dest");
-    DevBuf& bslot = c->buf("occ_slot");
-    bints.reserve(sizeof(double) * (size_t)std::max<int64_t>(1, n_vec2 * n_aux));
-    bdest.reserve(sizeof(int64_t) * (size_t)std::max<int64_t>(1, n_vec2));
-    bslot.reserve(sizeof(int64_t) * (size_t)std::max<int64_t>(1, nb));
-    if (n_vec2 * n_aux)
-      SA_CHECK_HIP(hipMemcpyAsync(bints.p, ints3c, sizeof(double) * (size_t)(n_vec2 * n_aux), hipMemcpyHostToDevice,
-                                  c->stream));
-    if (n_vec2)
-      SA_CHECK_HIP(hipMemcpyAsync(bdest.p, dest.data(), sizeof(int64_t) * (size_t)n_vec2, hipMemcpyHostToDevice,
-                                  c->stream));
-    SA_CHECK_HIP(hipMemcpyAsync(bslot.p, slot.data(), sizeof(int64_t) * (size_t)nb, hipMemcpyHostToDevice, c->stream));
-    bsm_ensure_capacity(U, total);
-    U->used = total;
-    fill_f64(c, U->data(), total, 0.0);
-    SA_CHECK_HIP(hipStreamSynchronize(c->stream));  // the host vectors above go out of scope
 
-    for (int64_t P = 0; P < n_aux; ++P) {  // occ_ri_k_step.cpp:100-118
-      if (n_vec2)
-        hipLaunchKernelGGL(k_occ_scatter, dim3((unsigned)ceil_div(n_vec2, 256)), dim3(256), 0, c->stream,
-                           (const double*)bints.as<double>() + P * n_vec2, (const int64_t*)bdest.as<int64_t>(),
-                           n_vec2, U->data());
-      hipLaunchKernelGGL(k_occ_block_norms, dim3((unsigned)ceil_div(nb, 4)), dim3(256), 0, c->stream,
-                         (const double*)U->data(), (const int64_t*)bslot.as<int64_t>(), (const int32_t*)U->d_rsz,
-                         (const int32_t*)U->d_csz, (const int32_t*)U->d_rscr, (const int32_t*)U->d_cscr, U->nrb, nb,
-                         U->thresh, U->d_off, U->d_norm);
-      SA_CHECK_HIP(hipGetLastError());
-      ok(sa_matmult(mui, U, 0, Cb, 0, thresh, 1.0, 0.0, opts));
-      ok(sa_bsm_calc_norms(mui));
-      ok(sa_matmult(ij, Cb, 1, mui, 0, thresh, 1.0, 0.0, opts));
-      ok(sa_bsm_copy(ijs, ij));
-      ok(sa_bsm_add_transpose(ijs, ij));
-      ok(sa_matmult(Ks, mui, 0, ijs, 0, thresh, 1.0, 1.0, opts));
+    // The team of occ_ri_k_step.cpp:90-126: P split as OpenMP's static
+    // schedule (the first n_aux % T members one iteration more), each member
+    // with its own context (stream), decompression buffer, mui, ij, ijs and
+    // K_sub; the members' K_sub are added in member order.
+    const int64_t T = std::max<int64_t>(1, std::min<int64_t>(teams > 0 ? teams : 8, std::max<int64_t>(n_aux, 1)));
+    ctxs.assign((size_t)T, nullptr);
+    ctxs[0] = c;
+    for (int64_t t = 1; t < T; ++t) ok(sa_ctx_create(c->device, &ctxs[(size_t)t]));
+    std::vector<sa_bsm_t> Ks((size_t)T);
+    for (int64_t t = 0; t < T; ++t) {
+      Ks[(size_t)t] = make(ctxs[(size_t)t], n_bf, n_occ, 0.0);
+      ok(sa_bsm_fill(Ks[(size_t)t], 0.0));
     }
-    ok(sa_bsm_add(Kb, Ks, 1.0));  // K_mu_i_bsm += K_mu_i_sub_bsm (occ_ri_k_step.cpp:122-125)
+    sa_bsm_t Kb = make(c, n_bf, n_occ, 0.0);
+    ok(sa_bsm_fill(Kb, 0.0));
+    ok(sa_ctx_synchronize(c));
+
+    std::vector<int> trc((size_t)T, SA_OK);
+    std::vector<std::string> terr((size_t)T);
+    auto member = [&](int64_t t) {
+      sa_ctx_t x = ctxs[(size_t)t];
+      std::vector<sa_bsm_t> own;
+      const int64_t q = n_aux / T, r = n_aux % T;
+      const int64_t P0 = t * q + std::min<int64_t>(t, r), np = q + (t < r ? 1 : 0);
+      trc[(size_t)t] = guard([&] {
+        SA_CHECK_HIP(hipSetDevice(x->device));
+        auto mk = [&](int64_t rr, int64_t qq, double th) {
+          sa_bsm_t m = nullptr;
+          ok(sa_bsm_create(x, rr, qq, bs, bs, th, &m));
+          own.push_back(m);
+          return m;
+        };
+        // occ_ri_k_step.cpp:84-98 (U's copy-in threshold is the reference's 1e-20)
+        sa_bsm_t Cb = mk(n_bf, n_occ, 0.0), U = mk(n_bf, n_bf, 1e-20);
+        sa_bsm_t mui = mk(n_bf, n_occ, 0.0), ij = mk(n_occ, n_occ, 0.0), ijs = mk(n_occ, n_occ, 0.0);
+        ok(sa_bsm_from_dense(Cb, mo, n_bf));
+        DevBuf& bints = x->buf("occ_ints");
+        DevBuf& bdest = x->buf("occ_dest");
+        DevBuf& bslot = x->buf("occ_slot");
+        bints.reserve(sizeof(double) * (size_t)std::max<int64_t>(1, n_vec2 * np));
+        bdest.reserve(sizeof(int64_t) * (size_t)std::max<int64_t>(1, n_vec2));
+        bslot.reserve(sizeof(int64_t) * (size_t)std::max<int64_t>(1, nb));
+        if (n_vec2 * np)
+          SA_CHECK_HIP(hipMemcpyAsync(bints.p, ints3c + P0 * n_vec2, sizeof(double) * (size_t)(n_vec2 * np),
+                                      hipMemcpyHostToDevice, x->stream));
+        if (n_vec2)
+          SA_CHECK_HIP(hipMemcpyAsync(bdest.p, dest.data(), sizeof(int64_t) * (size_t)n_vec2, hipMemcpyHostToDevice,
+                                      x->stream));
+        SA_CHECK_HIP(
+            hipMemcpyAsync(bslot.p, slot.data(), sizeof(int64_t) * (size_t)nb, hipMemcpyHostToDevice, x->stream));
+        bsm_ensure_capacity(U, total);
+        U->used = total;
+        fill_f64(x, U->data(), total, 0.0);
+        for (int64_t P = 0; P < np; ++P) {  // occ_ri_k_step.cpp:100-118
+          if (n_vec2)
+            hipLaunchKernelGGL(k_occ_scatter, dim3((unsigned)ceil_div(n_vec2, 256)), dim3(256), 0, x->stream,
+                               (const double*)bints.as<double>() + P * n_vec2, (const int64_t*)bdest.as<int64_t>(),
+                               n_vec2, U->data());
+          hipLaunchKernelGGL(k_occ_block_norms, dim3((unsigned)ceil_div(nb, 4)), dim3(256), 0, x->stream,
+                             (const double*)U->data(), (const int64_t*)bslot.as<int64_t>(), (const int32_t*)U->d_rsz,
+                             (const int32_t*)U->d_csz, (const int32_t*)U->d_rscr, (const int32_t*)U->d_cscr, U->nrb,
+                             nb, U->thresh, U->d_off, U->d_norm);
+          SA_CHECK_HIP(hipGetLastError());
+          ok(sa_matmult(mui, U, 0, Cb, 0, thresh, 1.0, 0.0, opts));
+          ok(sa_bsm_calc_norms(mui));
+          ok(sa_matmult(ij, Cb, 1, mui, 0, thresh, 1.0, 0.0, opts));
+          ok(sa_bsm_copy(ijs, ij));
+          ok(sa_bsm_add_transpose(ijs, ij));
+          ok(sa_matmult(Ks[(size_t)t], mui, 0, ijs, 0, thresh, 1.0, 1.0, opts));
+        }
+        SA_CHECK_HIP(hipStreamSynchronize(x->stream));
+      });
+      if (trc[(size_t)t] != SA_OK) terr[(size_t)t] = sa_last_error();
+      for (sa_bsm_t m : own) sa_bsm_destroy(m);
+    };
+    {
+      std::vector<std::thread> th;
+      for (int64_t t = 1; t < T; ++t) th.emplace_back(member, t);
+      member(0);
+      for (auto& h : th) h.join();
+    }
+    for (int64_t t = 0; t < T; ++t)
+      if (trc[(size_t)t] != SA_OK) {
+        set_error("%s", terr[(size_t)t].c_str());
+        throw Error{trc[(size_t)t]};
+      }
+    for (int64_t t = 0; t < T; ++t) ok(sa_bsm_add(Kb, Ks[(size_t)t], 1.0));  // K_mu_i_bsm += K_mu_i_sub_bsm
     for (int64_t i = 0; i < n_bf * n_occ; ++i) K[i] = 0.0;
     ok(sa_bsm_to_dense(Kb, K, n_bf));
   });
   for (sa_bsm_t m : mats) sa_bsm_destroy(m);
+  for (size_t t = 1; t < ctxs.size(); ++t)
+    if (ctxs[t]) sa_ctx_destroy(ctxs[t]);
   return rc;
 }
 
