@@ -6,15 +6,17 @@ Default workload (the headline metric): the RI-K step (n_bf=4000, n_aux=12000,
 n_occ=200, 300 random atoms).  A step is B' = X B followed by
 K = sum_{P,i} B'^T B' -- screening, task lists, block allocation and the
 grouped matrix-core GEMMs -- on inputs resident in HBM.  It is timed in split
-precision (FP16-slice Ozaki products, FP64 screening / norms / accumulation)
-and in plain FP64; value = dense-equivalent TFLOP/s of the split-precision
-step, with both step times, their ratio and the relative error of K against
-the FP64 result in the same JSON line.  The metric's inverse square root is
-set-up, timed separately.
+precision (INT8- or FP16-slice Ozaki products, FP64 screening / norms /
+accumulation) and in plain FP64; value = dense-equivalent TFLOP/s of the
+fastest split-precision form within 1e-10 of FP64, with both step times, their
+ratio and the relative error of K against the FP64 result in the same JSON
+line.  The metric's inverse square root is set-up, timed separately.
 
-Other workloads (--workload config1|config2|config3|config5): one screened
-block-sparse multiply C = A B per step (config3 = config1 inputs in split
-precision).
+Other workloads (--workload NAME): band (n 2048, 64-blocks, band |I-J| <= 4),
+atoms (n 12000, uneven 5-40 atom blocks, 10 % fill), band_split (the band
+inputs in split precision): one screened block-sparse multiply C = A B per
+step; hybrid (n 32768, 128-blocks, band 95): screened vs cost-model Strassen
+tree vs learned plan; occ: the occupied RI-K loop.
 
 Timing: HIP events on the library's stream around the timed steps; the
 dominant kernel's average launch time comes from the library's per-launch
@@ -68,7 +70,7 @@ def timed(ctx, fn, steps):
 
 
 # ----------------------------------------------------------------- RI-K --
-RIK_PROFILE = "profiles/gemm_ozaki_int8_rik_config4.md"
+RIK_PROFILE = "profiles/gemm_ozaki_int8_ri_k_step.md"
 
 
 def profiled_traffic(path):
@@ -311,14 +313,14 @@ def bench_hybrid(args):
     hybrid multiply into a pre-filled C, planning (dry run, features, network,
     plan) included; variants: plain screened multiply, the reference cost-model
     tree, the learned plan."""
-    w = synth.config5()
+    w = synth.hybrid_workload()
     n, bs, cutoff = w.n, w.uniform_bs, w.extra["cutoff"]
     ctx = sa.Context(0)
     A = sa.BSM.uniform(ctx, n, n, bs, bs, w.thresh).generate(1, w.a_scale)
     B = sa.BSM.uniform(ctx, n, n, bs, bs, w.thresh).generate(2, w.b_scale)
     Cm = sa.BSM.uniform(ctx, n, n, bs, bs, 0.0)
     Cm.fill(0.0)
-    log(f"config5: A, B generated ({A.info()[5] / 1e9:.1f} GB each), C pre-filled ({Cm.info()[5] / 1e9:.1f} GB)")
+    log(f"hybrid: A, B generated ({A.info()[5] / 1e9:.1f} GB each), C pre-filled ({Cm.info()[5] / 1e9:.1f} GB)")
     model = os.path.exists(sa.DEFAULT_PATH_MODEL)
     if model:
         sa.mlp_load(ctx, sa.DEFAULT_PATH_MODEL)
@@ -347,7 +349,7 @@ def bench_hybrid(args):
             fn()
         ms = timed(ctx, fn, args.steps)
         res[name] = ms
-        log(f"config5 {name}: {ms:.1f} ms per step")
+        log(f"hybrid {name}: {ms:.1f} ms per step")
     dense = 2.0 * n ** 3
     best = "learned" if model else "heuristic"
     rec = {
@@ -366,12 +368,12 @@ def bench_hybrid(args):
 
 # -------------------------------------------------------- screened GEMM --
 def bench_gemm(args):
-    if args.workload == "config3":
-        w = synth.config1()
-        w.name = "config3_ozaki_on_config1"
+    if args.workload == "band_split":
+        w = synth.band_workload()
+        w.name += "_split_fp16"
         opts = sa.default_opts(mode=sa.MODE_OZAKI, slices=args.slices_k, tile=args.tile)
     else:
-        w = {"config1": synth.config1, "config2": synth.config2, "config5": synth.config5}[args.workload]()
+        w = {"band": synth.band_workload, "atoms": synth.atoms_workload}[args.workload]()
         opts = sa.default_opts(tile=args.tile)
     ozaki = opts.mode == sa.MODE_OZAKI
     n = w.n
@@ -454,22 +456,22 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="rik", choices=["rik", "config1", "config2", "config3", "config5", "occ"])
+    ap.add_argument("--workload", default="rik", choices=["rik", "band", "atoms", "band_split", "hybrid", "occ"])
     ap.add_argument("--tile", type=int, default=-1)
-    ap.add_argument("--slices-k", type=int, default=4, help="split-precision slices (config3 workload)")
+    ap.add_argument("--slices-k", type=int, default=4, help="split-precision slices (band_split workload)")
     ap.add_argument("--forms", default="int8:5:5,fp16:4:4",
                     help="RI-K split-precision forms kind:slices_transform:slices_k[:tile_transform:tile_k] "
                          "(kind fp16|bf16|int8; tiles = split-precision kernel table ids, default chosen)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--n-occ", type=int, default=200, help="RI-K occupied orbitals (200 = the headline workload)")
-    ap.add_argument("--no-check", action="store_true", help="config5: skip the learned-vs-heuristic comparison")
+    ap.add_argument("--no-check", action="store_true", help="hybrid: skip the learned-vs-heuristic comparison")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1", "n_gpus": args.gpus}))
         sys.exit(2)
     if args.workload == "rik":
         bench_rik(args)
-    elif args.workload == "config5":
+    elif args.workload == "hybrid":
         bench_hybrid(args)
     elif args.workload == "occ":
         bench_occ(args)

@@ -105,21 +105,21 @@ class Workload:
         return int(self.rsz.sum())
 
 
-def config1(n: int = 2048, bs: int = 64, band: int = 4, decay: float = 0.5, thresh: float = 1e-10) -> Workload:
+def band_workload(n: int = 2048, bs: int = 64, band: int = 4, decay: float = 0.5, thresh: float = 1e-10) -> Workload:
     """n=2048, uniform 64x64 blocks, banded |I-J|<=4 (26.2 % fill),
     scale exp(-0.5|I-J|), threshold 1e-10, Strassen cutoff 512."""
     sz = uniform_sizes(n, bs)
     s = band_scales(len(sz), band, decay)
-    return Workload("config1_band_fp64", sz, sz, sz, s, s, thresh, uniform_bs=bs,
+    return Workload(f"band_n{n}_bs{bs}_w{band}", sz, sz, sz, s, s, thresh, uniform_bs=bs,
                     extra={"band": band, "decay": decay, "cutoff": 512})
 
 
-def config5(n: int = 32768, bs: int = 128, band: int = 95, decay: float = 0.05, thresh: float = 1e-10) -> Workload:
+def hybrid_workload(n: int = 32768, bs: int = 128, band: int = 95, decay: float = 0.05, thresh: float = 1e-10) -> Workload:
     """n=32768, uniform 128x128 blocks, band |I-J|<=95 (60.7 % fill),
     scale exp(-0.05|I-J|), Strassen depth <= 4 (cutoff 2048)."""
     sz = uniform_sizes(n, bs)
     s = band_scales(len(sz), band, decay)
-    return Workload("config5_hybrid", sz, sz, sz, s, s, thresh, uniform_bs=bs,
+    return Workload(f"hybrid_n{n}_bs{bs}_band{band}", sz, sz, sz, s, s, thresh, uniform_bs=bs,
                     extra={"band": band, "decay": decay, "cutoff": 2048})
 
 
@@ -142,7 +142,7 @@ def atom_sizes(n_target: int, rng: np.random.Generator, table=SHELL_SIZES) -> np
     return np.asarray(sizes, dtype=np.int32)
 
 
-def config2(n: int = 12000, fill: float = 0.10, decay: float = 0.3, thresh: float = 1e-10,
+def atoms_workload(n: int = 12000, fill: float = 0.10, decay: float = 0.3, thresh: float = 1e-10,
             seed: int = 3) -> Workload:
     """Uneven atom-shaped blocks (5-40, seeded shell-size table), n about 12000,
     ~10 % block fill by distance cutoff over random 3D atom positions, block
@@ -156,12 +156,12 @@ def config2(n: int = 12000, fill: float = 0.10, decay: float = 0.3, thresh: floa
     rc = float(np.quantile(d.ravel(), fill))
     s = np.exp(-decay * d)
     s[d > rc] = 0.0
-    return Workload("config2_uneven_atoms", sz, sz, sz, s, s, thresh,
+    return Workload(f"uneven_atoms_n{n}_fill{fill:g}", sz, sz, sz, s, s, thresh,
                     extra={"natoms": na, "rc": rc, "box": box, "decay": decay, "fill": fill})
 
 
 def small_uneven(n: int = 600, fill: float = 0.3, seed: int = 7, thresh: float = 1e-10) -> Workload:
-    w = config2(n=n, fill=fill, seed=seed, thresh=thresh)
+    w = atoms_workload(n=n, fill=fill, seed=seed, thresh=thresh)
     w.name = "small_uneven"
     return w
 
@@ -244,7 +244,7 @@ class RIKWorkload:
 
 def rik_config(natoms=300, n_bf=4000, n_aux=12000, n_occ=200, occ_chunk=50, p_group=160, mu_group=100,
                fill=0.35, decay=0.3, alpha=0.5, ridge=0.05, jitter=0.35, thresh=1e-10, seed=4,
-               name="config4_ri_k_step") -> RIKWorkload:
+               name="ri_k_step_300_atoms") -> RIKWorkload:
     """RI-K step: 300 random atoms, n_bf = 4000, n_aux = 12000, n_occ = 200.
     Atoms are ordered along a Morton curve and grouped into P-groups of about
     p_group aux functions and mu-groups of about mu_group basis functions; B

@@ -53,7 +53,7 @@ def test_generator_matches_host_definition():
     o = oracle.OBsm(w.rsz, w.ksz, 1e-10).from_dense(a)
     assert np.array_equal(d.to_dense(), a)
     assert_same_meta(o, d)
-    cfg = synth.config1(n=512, bs=64)
+    cfg = synth.band_workload(n=512, bs=64)
     d = sa.BSM.uniform(ctx(), 512, 512, 64, 64, 1e-10).generate(1, cfg.a_scale)
     assert np.array_equal(d.to_dense(), synth.dense_from_scales(1, cfg.rsz, cfg.ksz, cfg.a_scale))
 

@@ -85,7 +85,7 @@ def test_pipelined_tiles_transposes(tile, ta, tb):
 
 
 def test_binding_threshold_screens_tasks():
-    cfg = synth.config1(n=1024, bs=64, band=6, decay=2.5)
+    cfg = synth.band_workload(n=1024, bs=64, band=6, decay=2.5)
     a = synth.dense_from_scales(1, cfg.rsz, cfg.ksz, cfg.a_scale)
     b = synth.dense_from_scales(2, cfg.ksz, cfg.csz, cfg.b_scale)
     oA, dA = pair_uniform(a, 64, 64, 1e-12)
@@ -104,9 +104,9 @@ def test_empty_operands():
     assert n == 0 and dC.nalloc() == 0
 
 
-def test_config1_full_size():
+def test_band_full_size():
     """n=2048, 64x64 blocks, band |I-J|<=4, exp(-0.5|I-J|) scales, thresh 1e-10."""
-    cfg = synth.config1()
+    cfg = synth.band_workload()
     oA = oracle.OBsm.uniform(2048, 2048, 64, 64, cfg.thresh).from_dense(
         synth.dense_from_scales(1, cfg.rsz, cfg.ksz, cfg.a_scale))
     oB = oracle.OBsm.uniform(2048, 2048, 64, 64, cfg.thresh).from_dense(
@@ -120,7 +120,7 @@ def test_config1_full_size():
 
 
 def test_uneven_atoms_moderate():
-    w = synth.config2(n=2500, fill=0.1, seed=9)
+    w = synth.atoms_workload(n=2500, fill=0.1, seed=9)
     a = synth.dense_from_scales(1, w.rsz, w.ksz, w.a_scale)
     b = synth.dense_from_scales(2, w.ksz, w.csz, w.b_scale)
     oA, dA = pair_blocked(a, w.rsz, w.ksz, w.thresh)
@@ -152,7 +152,7 @@ def test_kwindows_match_single_launch(monkeypatch, beta):
     """A 1 MB operand budget splits the band product into many k-windows; the
     task list is unchanged and C stays within the FP64 bar, with existing C
     blocks (beta = 1) accumulated only in their first window."""
-    cfg = synth.config1(n=1024, bs=64, band=6)
+    cfg = synth.band_workload(n=1024, bs=64, band=6)
     a = synth.dense_from_scales(1, cfg.rsz, cfg.ksz, cfg.a_scale)
     b = synth.dense_from_scales(2, cfg.ksz, cfg.csz, cfg.b_scale)
     c0 = synth.dense_from_scales(3, cfg.rsz, cfg.csz, cfg.a_scale)

@@ -191,7 +191,7 @@ def test_trained_model_gain_over_heuristic():
         pytest.skip("no trained path model in the tree")
     c = sa.Context(0)
     sa.mlp_load(c, sa.DEFAULT_PATH_MODEL)
-    w = synth.config5(n=8192, bs=128, band=24)
+    w = synth.hybrid_workload(n=8192, bs=128, band=24)
     A = sa.BSM.uniform(c, w.n, w.n, 128, 128, w.thresh).generate(1, w.a_scale)
     B = sa.BSM.uniform(c, w.n, w.n, 128, 128, w.thresh).generate(2, w.b_scale)
     Cm = sa.BSM.uniform(c, w.n, w.n, 128, 128, 0.0)

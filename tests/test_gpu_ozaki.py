@@ -36,8 +36,8 @@ def run(oA, dA, ta, oB, dB, tb, oC, dC, thresh, opts, alpha=1.0, beta=0.0):
 @pytest.mark.parametrize("slices,stype", [(4, sa.SLICE_FP16), (5, sa.SLICE_FP16), (5, sa.SLICE_BF16),
                                           (6, sa.SLICE_BF16), (4, sa.SLICE_INT8), (5, sa.SLICE_INT8),
                                           (6, sa.SLICE_INT8)])
-def test_config1_like_error_per_block_row(slices, stype):
-    cfg = synth.config1(n=1024, bs=64)
+def test_band_error_per_block_row(slices, stype):
+    cfg = synth.band_workload(n=1024, bs=64)
     a = synth.dense_from_scales(1, cfg.rsz, cfg.ksz, cfg.a_scale)
     b = synth.dense_from_scales(2, cfg.ksz, cfg.csz, cfg.b_scale)
     oA, dA = pair_uniform(a, 64, 64, cfg.thresh)
@@ -151,7 +151,7 @@ def test_unsupported_slice_count_rejected():
 
 def test_kwindows_split_precision(monkeypatch):
     """k-windows under the split-precision kernel: same task list, same bar."""
-    cfg = synth.config1(n=1024, bs=64)
+    cfg = synth.band_workload(n=1024, bs=64)
     a = synth.dense_from_scales(1, cfg.rsz, cfg.ksz, cfg.a_scale)
     b = synth.dense_from_scales(2, cfg.ksz, cfg.csz, cfg.b_scale)
     oA, dA = pair_uniform(a, 64, 64, cfg.thresh)
