@@ -943,8 +943,6 @@ int64_t ozaki_multiply(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, double
   DevBuf& bprods = c->buf("oz_prods");
   bouts.reserve(sizeof(GemmOut) * (size_t)(R.nouts + 1));
   bprods.reserve(sizeof(OzProd) * (size_t)(R.ntasks + 1));
-  const std::vector<int32_t>& ksz_h = ta ? A->rsz : A->csz;
-  (void)ksz_h;
   OzTables t;
   t.cnt = R.cnt; t.seg = R.seg; t.need_scan = R.need_scan; t.out_scan = R.out_scan; t.kb = R.kb;
   t.coff = C->d_off; t.cnorm = C->d_norm; t.carena = C->data(); t.crsz = C->d_rsz; t.ccsz = C->d_csz;
