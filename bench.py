@@ -132,7 +132,10 @@ def bench_rik(args):
     dBp = sa.BSM(ctx, w.p_sizes, bcs, 0.0)
     dK = sa.BSM(ctx, w.mu_sizes, w.mu_sizes, 0.0)
     o64 = sa.default_opts()
-    kinds = {"fp16": sa.SLICE_FP16, "bf16": sa.SLICE_BF16, "int8": sa.SLICE_INT8}
+    # kind suffix "s": the transform's operands sliced with shared exponents (one
+    # int32 accumulation and one FP64 fold per output instead of one per product)
+    kinds = {"fp16": sa.SLICE_FP16, "bf16": sa.SLICE_BF16, "int8": sa.SLICE_INT8,
+             "fp16s": sa.SLICE_FP16, "bf16s": sa.SLICE_BF16, "int8s": sa.SLICE_INT8}
     forms = []
     for f in args.forms.split(","):
         parts = f.split(":")
@@ -172,7 +175,8 @@ def bench_rik(args):
     results = {}
     for kind, s_t, s_k, tiles in forms:
         name = f"{kind}_{s_t}_{s_k}" + ("" if tiles == (-1, -1) else f"_t{tiles[0]}_{tiles[1]}")
-        oz_t = sa.default_opts(mode=sa.MODE_OZAKI, slices=s_t, slice_type=kinds[kind], tile=tiles[0])
+        oz_t = sa.default_opts(mode=sa.MODE_OZAKI, slices=s_t, slice_type=kinds[kind], tile=tiles[0],
+                               shared_exp=1 if kind.endswith("s") else 0)
         oz_k = sa.default_opts(mode=sa.MODE_OZAKI, slices=s_k, slice_type=kinds[kind], tile=tiles[1])
 
         def stepmix():
@@ -187,7 +191,7 @@ def bench_rik(args):
                                                                        "rik_transform", "rik_k")}
         err = rel(dK.to_dense(), K64)
         pairs = s_t * (s_t + 1) // 2 * f_tr + s_k * (s_k + 1) // 2 * f_k
-        peak = PEAK_INT8 if kind == "int8" else PEAK_FP16
+        peak = PEAK_INT8 if kind.startswith("int8") else PEAK_FP16
         ach = pairs / (goz_ms / args.steps * 1e-3) / 1e12
         results[name] = {"slice_type": kind, "slices_transform": s_t, "slices_k": s_k, "step_ms": round(ms, 3),
                          "speedup_vs_fp64_gpu": round(ms64 / ms, 3), "rel_err_K_vs_fp64": err,
@@ -461,7 +465,8 @@ def main():
     ap.add_argument("--slices-k", type=int, default=4, help="split-precision slices (band_split workload)")
     ap.add_argument("--forms", default="int8:5:5,fp16:4:4",
                     help="RI-K split-precision forms kind:slices_transform:slices_k[:tile_transform:tile_k] "
-                         "(kind fp16|bf16|int8; tiles = split-precision kernel table ids, default chosen)")
+                         "(kind fp16|bf16|int8, suffix s for shared exponents in the transform, e.g. int8s; "
+                         "tiles = split-precision kernel table ids, default chosen)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--n-occ", type=int, default=200, help="RI-K occupied orbitals (200 = the headline workload)")
     ap.add_argument("--no-check", action="store_true", help="hybrid: skip the learned-vs-heuristic comparison")

@@ -53,7 +53,11 @@ typedef struct {
   int32_t tile;        /* FP64: GEMM tile config id (sa_ctx_gemm_config), -1 = autotuned;
                           Ozaki: kernel table id, ignored unless it runs this slice count/type */
   int32_t scheduler;   /* SA_SCHED_* (hybrid Strassen only) */
-  int32_t reserved;
+  int32_t shared_exp;  /* Ozaki: 0 (default) = power-of-two scale per row / column of each
+                          block; 1 = per row of op(A) and column of op(B) across all inner
+                          blocks, so an output's products share one int32 accumulation and
+                          fold into FP64 once (fewer folds for short inner blocks; small
+                          blocks then carry fewer significant bits) */
   int64_t cutoff;      /* min_size_for_strassen (main.cpp:9), elements */
   const uint8_t* tree; /* SA_SCHED_TREE: Strassen-root bits in dry-run layout */
   const uint8_t* tree_dense; /* SA_SCHED_TREE (optional): dense-root bits, same layout; a

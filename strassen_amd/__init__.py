@@ -47,7 +47,7 @@ INT = C.c_int
 
 class Opts(C.Structure):
     _fields_ = [("mode", I32), ("slices", I32), ("slice_type", I32), ("tile", I32),
-                ("scheduler", I32), ("reserved", I32), ("cutoff", I64), ("tree", P), ("tree_dense", P)]
+                ("scheduler", I32), ("shared_exp", I32), ("cutoff", I64), ("tree", P), ("tree_dense", P)]
 
 
 # name -> (restype, argtypes); every function declared in include/strassen_amd.h

@@ -91,7 +91,7 @@ struct OzProd {
   int32_t nks;        // MFMA k-steps
   int32_t nvba;       // 16-row blocks of the op(A) block
   int32_t nvbb;       // 16-column blocks of the op(B) block
-  int32_t pad;
+  int32_t chain;      // 1: same row/column exponents as the previous product of its output
 };
 
 // --------------------------------------------------------------- slicing --
@@ -145,10 +145,14 @@ __global__ void k_slice_need(SliceArgs s, int S, int kstep, int esize, int64_t* 
 // the digit at position j < S-1 is ((m >> beta j) & (2^beta - 1)) - 2^(beta-1)
 // and the leading one m >> beta (S-1), the same digits as peeling n from its
 // least significant end.
+// mode 0: exponents and digits; 1: exponents only; 2: digits with the
+// exponents already in exps (shared by all blocks of a vector's line)
+constexpr int kNoExp = INT32_MIN / 2;  // exponent of an all-zero vector before sharing
+
 template <int KIND>
 __global__ void __launch_bounds__(64) k_slice(SliceArgs s, int S, const int64_t* __restrict__ sl_off,
                                               const int64_t* __restrict__ ex_off, int64_t ngroups,
-                                              uint8_t* __restrict__ slices, int32_t* __restrict__ exps) {
+                                              uint8_t* __restrict__ slices, int32_t* __restrict__ exps, int mode) {
   constexpr int BETA = oz_beta(KIND), TS = oz_tshift(KIND);
   constexpr int KS = oz_kstep(KIND), ES = oz_esize(KIND), EL = 16 / ES;
   __shared__ double T[16][KS + 1];  // one k-step tile; the pad keeps the 16 rows on distinct banks
@@ -196,19 +200,25 @@ __global__ void __launch_bounds__(64) k_slice(SliceArgs s, int S, const int64_t*
     }
   };
 
-  double mx = 0.0;
-  for (int ks = 0; ks < nks; ++ks) {
-    stage(ks);
-    __syncthreads();
-#pragma unroll
-    for (int t = 0; t < EL; ++t) mx = fmax(mx, fabs(T[vl][kg * EL + t]));
-    __syncthreads();
-  }
-  mx = fmax(mx, __shfl_xor(mx, 16, 64));
-  mx = fmax(mx, __shfl_xor(mx, 32, 64));
   int e = 0;
-  if (mx > 0.0) (void)frexp(mx, &e);  // mx < 2^e
-  if (kg == 0) exps[ex_off[b] + v0 + vl] = e;
+  if (mode == 2) {
+    e = exps[ex_off[b] + v0 + vl];  // >= this block's own exponent: digits stay in range
+  } else {
+    double mx = 0.0;
+    for (int ks = 0; ks < nks; ++ks) {
+      stage(ks);
+      __syncthreads();
+#pragma unroll
+      for (int t = 0; t < EL; ++t) mx = fmax(mx, fabs(T[vl][kg * EL + t]));
+      __syncthreads();
+    }
+    mx = fmax(mx, __shfl_xor(mx, 16, 64));
+    mx = fmax(mx, __shfl_xor(mx, 32, 64));
+    if (mx > 0.0) (void)frexp(mx, &e);  // mx < 2^e
+    else if (mode == 1) e = kNoExp;     // a zero vector does not raise its line's exponent
+    if (kg == 0) exps[ex_off[b] + v0 + vl] = e;
+    if (mode == 1) return;
+  }
 
   const int64_t mask = ((int64_t)1 << BETA) - 1, half = (int64_t)1 << (BETA - 1);
   int64_t H = 0;
@@ -218,8 +228,7 @@ __global__ void __launch_bounds__(64) k_slice(SliceArgs s, int S, const int64_t*
     __syncthreads();
     int64_t mv[EL];
 #pragma unroll
-    for (int t = 0; t < EL; ++t)
-      mv[t] = (mx > 0.0 ? (int64_t)rint(ldexp(T[vl][kg * EL + t], BETA * S - TS - e)) : 0) + H;
+    for (int t = 0; t < EL; ++t) mv[t] = (int64_t)rint(ldexp(T[vl][kg * EL + t], BETA * S - TS - e)) + H;
     __syncthreads();
     for (int p = 0; p < S; ++p) {
       const int sh = BETA * (S - 1 - p);
@@ -236,6 +245,36 @@ __global__ void __launch_bounds__(64) k_slice(SliceArgs s, int S, const int64_t*
         }
       }
       *reinterpret_cast<uint4*>(out + (int64_t)p * pstride + (int64_t)ks * 1024) = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+  }
+}
+
+// Shared exponents: every vector of a line (an op(A) row across all its
+// blocks of one block row, an op(B) column across all blocks of one block
+// column) gets the largest of its blocks' exponents, so all products of an
+// output block scale each row and column alike and their digit products can
+// share the int32 accumulators.  One workgroup per line.
+__global__ void __launch_bounds__(256) k_exp_share(SliceArgs s, const int64_t* __restrict__ ex_off,
+                                                   int32_t* __restrict__ exps) {
+  const int line = blockIdx.x;
+  const int32_t nrb = s.nrb, ncb = (int32_t)(s.nb / s.nrb);
+  // lines run along the op-vector index: op(A) rows = A rows (block row) unless
+  // transposed; op(B) columns = B columns (block column) unless transposed
+  const bool by_row = s.role_b ? (s.trans != 0) : (s.trans == 0);
+  const int32_t nblk = by_row ? ncb : nrb;
+  const int nv = by_row ? s.rsz[line] : s.csz[line];
+  const int nvp = (int)ceil_div(nv, 16) * 16;
+  auto blk = [&](int32_t i) { return by_row ? (int64_t)line + (int64_t)i * nrb : (int64_t)i + (int64_t)line * nrb; };
+  for (int v = threadIdx.x; v < nvp; v += blockDim.x) {
+    int e = kNoExp;
+    for (int32_t i = 0; i < nblk; ++i) {
+      const int64_t b = blk(i);
+      if (s.off[b] >= 0) e = max(e, exps[ex_off[b] + v]);
+    }
+    if (e == kNoExp) e = 0;
+    for (int32_t i = 0; i < nblk; ++i) {
+      const int64_t b = blk(i);
+      if (s.off[b] >= 0) exps[ex_off[b] + v] = e;
     }
   }
 }
@@ -269,6 +308,7 @@ struct OzTables {
   const int32_t* bex;
   const int64_t* bex_off;
   int32_t kstep;
+  int32_t shared;  // operands sliced with shared exponents: the products of an output chain
 };
 
 __global__ void k_oz_records(OzTables t, GemmOut* __restrict__ outs, OzProd* __restrict__ prods) {
@@ -307,7 +347,7 @@ __global__ void k_oz_records(OzTables t, GemmOut* __restrict__ outs, OzProd* __r
     pr.nks = (int32_t)ceil_div(t.ksz[kb], t.kstep);
     pr.nvba = (int32_t)ceil_div(o.m, 16);
     pr.nvbb = (int32_t)ceil_div(o.n, 16);
-    pr.pad = 0;
+    pr.chain = (t.shared && p > seg) ? 1 : 0;
     prods[p] = pr;
   }
 }
@@ -501,6 +541,7 @@ __global__ void __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_
       load_exps(pr, er, ec);
       gload(pr, 0, I0{});
       __syncthreads();
+      int run = 0;  // k-steps in the int32 / FP32 accumulators since the last fold
       // one stage computing from buffer P while the next loads into P ^ 1;
       // true at the end of the stream
       auto step = [&](auto par) -> bool {
@@ -521,8 +562,14 @@ __global__ void __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_
           gload(pn, nks, std::integral_constant<int, P ^ 1>{});
         }
         compute(par);
-        const int done = ks + 1;
-        if (np != p || (done & (FULL - 1)) == 0) fold();
+        // fold at the end of a product unless the next one chains (same
+        // exponents), and after FULL k-steps of one accumulation in any case
+        ++run;
+        const bool chained = np != p && has_next && pn.chain;
+        if ((np != p && !chained) || run == FULL) {
+          fold();
+          run = 0;
+        }
         if (!has_next) return true;
         if (np != p) {
 #pragma unroll
@@ -839,7 +886,9 @@ struct Sliced {
   const int64_t* ex_off;
 };
 
-static Sliced slice_operand(sa_context* c, sa_bsm* X, bool role_b, bool trans, int S, int kind, const char* tag) {
+// shared: exponents per op-vector line across blocks (k_exp_share), else per block
+static Sliced slice_operand(sa_context* c, sa_bsm* X, bool role_b, bool trans, int S, int kind, bool shared,
+                            const char* tag) {
   const int64_t nb = X->nb();
   std::string t(tag);
   DevBuf& bso = c->buf((t + "_sloff").c_str());
@@ -867,12 +916,26 @@ static Sliced slice_operand(sa_context* c, sa_bsm* X, bool role_b, bool trans, i
       const int64_t ng = nex / 16;  // 16-vector groups (nex: padded vectors of all blocks)
       const dim3 g((unsigned)std::max<int64_t>(1, ng)), t(64);
       const int64_t *so = (const int64_t*)bso.p, *eo = (const int64_t*)beo.p;
-      if (kind == 2)
-        hipLaunchKernelGGL(k_slice<2>, g, t, 0, c->stream, a, S, so, eo, ng, bsl.as<uint8_t>(), bex.as<int32_t>());
-      else if (kind == 1)
-        hipLaunchKernelGGL(k_slice<1>, g, t, 0, c->stream, a, S, so, eo, ng, bsl.as<uint8_t>(), bex.as<int32_t>());
-      else
-        hipLaunchKernelGGL(k_slice<0>, g, t, 0, c->stream, a, S, so, eo, ng, bsl.as<uint8_t>(), bex.as<int32_t>());
+      auto run = [&](int mode) {
+        if (kind == 2)
+          hipLaunchKernelGGL(k_slice<2>, g, t, 0, c->stream, a, S, so, eo, ng, bsl.as<uint8_t>(), bex.as<int32_t>(),
+                             mode);
+        else if (kind == 1)
+          hipLaunchKernelGGL(k_slice<1>, g, t, 0, c->stream, a, S, so, eo, ng, bsl.as<uint8_t>(), bex.as<int32_t>(),
+                             mode);
+        else
+          hipLaunchKernelGGL(k_slice<0>, g, t, 0, c->stream, a, S, so, eo, ng, bsl.as<uint8_t>(), bex.as<int32_t>(),
+                             mode);
+      };
+      if (shared) {
+        run(1);
+        const bool by_row = role_b ? trans : !trans;
+        hipLaunchKernelGGL(k_exp_share, dim3((unsigned)(by_row ? X->nrb : X->ncb)), dim3(256), 0, c->stream, a, eo,
+                           bex.as<int32_t>());
+        run(2);
+      } else {
+        run(0);
+      }
     }
   }
   SA_CHECK_HIP(hipGetLastError());
@@ -936,8 +999,10 @@ int64_t ozaki_multiply(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, double
   // slice operands; share when both roles read the same vectors of one matrix
   const sa_bsm* ra = A->base ? A->base : A;
   const sa_bsm* rb = B->base ? B->base : B;
-  const Sliced sa_ = slice_operand(c, A, false, ta, S, kind, "oz_a");
-  const Sliced sb_ = (ra == rb && ta != tb && A == B) ? sa_ : slice_operand(c, B, true, tb, S, kind, "oz_b");
+  const bool shared = o.shared_exp != 0;
+  const Sliced sa_ = slice_operand(c, A, false, ta, S, kind, shared, "oz_a");
+  const Sliced sb_ =
+      (ra == rb && ta != tb && A == B) ? sa_ : slice_operand(c, B, true, tb, S, kind, shared, "oz_b");
 
   DevBuf& bouts = c->buf("mm_outs");
   DevBuf& bprods = c->buf("oz_prods");
@@ -953,6 +1018,7 @@ int64_t ozaki_multiply(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, double
   t.asl = sa_.sl; t.asl_off = sa_.sl_off; t.aex = sa_.ex; t.aex_off = sa_.ex_off;
   t.bsl = sb_.sl; t.bsl_off = sb_.sl_off; t.bex = sb_.ex; t.bex_off = sb_.ex_off;
   t.kstep = oz_kstep(kind);
+  t.shared = shared ? 1 : 0;
   hipLaunchKernelGGL(k_oz_records, dim3((unsigned)ceil_div(R.nc, 128)), dim3(128), 0, c->stream, t,
                      bouts.as<GemmOut>(), bprods.as<OzProd>());
 

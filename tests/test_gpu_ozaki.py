@@ -84,6 +84,31 @@ def test_deep_k_fold_schedule(stype, slices):
     assert rel_fro(got, ref) < TOL
 
 
+@pytest.mark.parametrize("slices,stype", [(5, sa.SLICE_INT8), (4, sa.SLICE_FP16), (5, sa.SLICE_BF16)])
+@pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1)])
+def test_shared_exponents(slices, stype, ta, tb):
+    """Exponents shared along op(A) rows / op(B) columns across inner blocks:
+    the products of an output chain in one accumulation (many short products
+    of ragged blocks with decaying norms, so the shared scale is not every
+    block's own; vectors that are zero in one block)."""
+    rng = np.random.default_rng(40 + slices + 3 * ta + 5 * tb)
+    rsz, ksz, csz = [24, 40, 33], [17, 64, 9, 48, 30, 21, 64, 5], [36, 28]
+    ni, nk, nj = sum(rsz), sum(ksz), sum(csz)
+    kb = np.repeat(np.arange(len(ksz)), ksz)
+    a = rng.standard_normal((ni, nk)) * np.exp(-0.7 * kb)[None, :]
+    b = rng.standard_normal((nk, nj)) * np.exp(-0.4 * kb)[:, None]
+    a *= 1e-6  # below 1: a zero vector must not lift its line's exponent to 0
+    a[:8, :17] = 0.0
+    b[81:90, :5] = 0.0
+    a_, b_ = (a.T.copy(), ksz, rsz) if ta else (a, rsz, ksz), (b.T.copy(), csz, ksz) if tb else (b, ksz, csz)
+    oA, dA = pair_blocked(np.asfortranarray(a_[0]), a_[1], a_[2], 0.0)
+    oB, dB = pair_blocked(np.asfortranarray(b_[0]), b_[1], b_[2], 0.0)
+    oC, dC = empty_pair_blocked(rsz, csz)
+    ref, got = run(oA, dA, ta, oB, dB, tb, oC, dC, 0.0,
+                   sa.default_opts(mode=sa.MODE_OZAKI, slices=slices, slice_type=stype, shared_exp=1))
+    assert rel_fro(got, ref) < TOL
+
+
 DEEP = [(12, 3, sa.SLICE_FP16), (13, 4, sa.SLICE_FP16), (14, 5, sa.SLICE_FP16), (15, 3, sa.SLICE_FP16),
         (16, 4, sa.SLICE_FP16), (17, 5, sa.SLICE_FP16), (18, 4, sa.SLICE_BF16), (19, 5, sa.SLICE_BF16),
         (20, 4, sa.SLICE_BF16), (21, 5, sa.SLICE_BF16), (22, 4, sa.SLICE_INT8), (23, 5, sa.SLICE_INT8),
