@@ -84,6 +84,62 @@ def test_deep_k_fold_schedule(stype, slices):
     assert rel_fro(got, ref) < TOL
 
 
+@pytest.mark.parametrize("slices", [4, 5, 6])
+def test_int8_extreme_digits_short_products(slices):
+    """Entries whose 5-slice INT8 digits sit at the ends of their ranges
+    (leading digit +-63/64, the others -128 or 127) in products of 1 to 4
+    k-steps, against a host restatement of the split product (exact integer
+    slice sums, FP64 only in the fold)."""
+    rng = np.random.default_rng(50 + slices)
+    pats = np.array([272722001792, 272722001791, -272738844800, -272738844801], dtype=np.float64) * 2.0 ** -38
+    ksz = [192, 130, 64, 200, 17]
+    rsz, csz = [48, 80], [64, 33]
+    a = rng.choice(pats, (sum(rsz), sum(ksz)))
+    b = rng.choice(pats, (sum(ksz), sum(csz)))
+    a[:, :192] = pats[0]  # one product of 3 k-steps with every digit pair at its largest
+    b[:192, :] = pats[0]
+    oA, dA = pair_blocked(a, rsz, ksz, 0.0)
+    oB, dB = pair_blocked(b, ksz, csz, 0.0)
+    oC, dC = empty_pair_blocked(rsz, csz)
+    ref, got = run(oA, dA, 0, oB, dB, 0, oC, dC, 0.0,
+                   sa.default_opts(mode=sa.MODE_OZAKI, slices=slices, slice_type=sa.SLICE_INT8))
+    # the scheme itself: exact integer slice products, FP64 only in the fold
+    assert rel_fro(got, int8_slice_model(a, rsz, ksz, b, csz, slices)) < 1e-14
+    if slices >= 5:  # 4 slices round these coherent inputs to 2^-31: ~2e-9 against FP64
+        assert rel_fro(got, ref) < TOL
+
+
+def int8_slice_model(a, rsz, ksz, b, csz, S):
+    """Split product with INT8 digits restated on the host: per row of an
+    A block / column of a B block the scale 2^e above its largest entry,
+    n = rint(x 2^(8S-2-e)), digits of n + H (leading one m >> 8(S-1), the
+    others byte - 128), slice pairs p + q <= S - 1 summed exactly in int64,
+    folded into FP64 per product in k order."""
+    H = sum(128 << (8 * j) for j in range(S - 1))
+
+    def digits(x):  # vectors along axis 0
+        mx = np.abs(x).max(axis=1)
+        e = np.where(mx > 0, np.frexp(mx)[1], 0)
+        m = np.rint(np.ldexp(x, (8 * S - 2 - e)[:, None])).astype(np.int64) + H
+        return [m >> (8 * (S - 1))] + [((m >> (8 * j)) & 255) - 128 for j in range(S - 2, -1, -1)], e
+
+    re_, ke, ce = np.cumsum([0] + rsz), np.cumsum([0] + ksz), np.cumsum([0] + csz)
+    C = np.zeros((re_[-1], ce[-1]))
+    for i in range(len(rsz)):
+        for j in range(len(csz)):
+            acc = np.zeros((rsz[i], csz[j]))
+            for k in range(len(ksz)):
+                da, er = digits(a[re_[i]:re_[i + 1], ke[k]:ke[k + 1]])
+                db, ec = digits(b[ke[k]:ke[k + 1], ce[j]:ce[j + 1]].T)
+                t = np.zeros((rsz[i], csz[j]))
+                for w in range(S):
+                    sw = sum(da[p] @ db[w - p].T for p in range(w + 1))
+                    t += np.ldexp(sw.astype(np.float64), -8 * w)
+                acc += np.ldexp(t, er[:, None] + ec[None, :] + 4 - 16)
+            C[re_[i]:re_[i + 1], ce[j]:ce[j + 1]] = acc
+    return C
+
+
 @pytest.mark.parametrize("slices,stype", [(5, sa.SLICE_INT8), (4, sa.SLICE_FP16), (5, sa.SLICE_BF16)])
 @pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1)])
 def test_shared_exponents(slices, stype, ta, tb):
