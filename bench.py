@@ -21,7 +21,9 @@ tree vs learned plan; occ: the occupied RI-K loop.
 Timing: HIP events on the library's stream around the timed steps; the
 dominant kernel's average launch time comes from the library's per-launch
 HIP events (same stream).  The CPU baseline is the oracle (single-threaded C
-restatement) on a bounded sample, scaled by algorithmic flops.
+restatement) on a bounded sample, scaled by algorithmic flops; where the
+reference's own build (oracle/_ref: its templates, MKL, OpenMP) is present,
+cpu_reference times it on the host's cores as well.
 """
 from __future__ import annotations
 
@@ -109,6 +111,68 @@ def cpu_baseline_rik(w, X, budget_s=15.0):
         if dt > budget_s / 3 or g >= len(w.p_sizes):
             return dt, fl, g, rows
         g = min(len(w.p_sizes), max(g + 1, int(g * min(4.0, budget_s / 3 / max(dt, 1e-3)))))
+
+
+REF_THREADS = 16  # host cores that go with one GPU on the benchmark machines
+
+
+def ref_cores():
+    return max(1, min(REF_THREADS, os.cpu_count() or 1))
+
+
+def load_ref():
+    """oracle/_ref (the reference's own templates, Intel MKL, OpenMP), or None.
+    MKL stays sequential inside the reference's OpenMP loops over blocks."""
+    os.environ.setdefault("MKL_NUM_THREADS", "1")
+    import oracle
+
+    return oracle.ref() if oracle.ref_available() else None
+
+
+def cpu_reference_rik(ctx, w, X, budget_s=6.0):
+    """The reference's screened multiply (blocksparsematrix.hpp:292-414: MKL
+    dgemm per block, OpenMP over C super-blocks on ref_cores() threads) on a
+    uniform-block stand-in of the transform sample -- rows of X against B's
+    first column block, square blocks of the mean P-group size, since the
+    reference takes uniform blocks only -- grown until about budget_s.  The
+    task list of that product (for its algorithmic flops) comes from this
+    library, whose screening matches the reference's bit for bit."""
+    import oracle
+
+    R = load_ref()
+    if R is None:
+        return None
+    cores = R.ref_set_threads(ref_cores())
+    bs = int(round(float(np.mean(w.p_sizes))))
+    bcs = w.b_col_sizes
+    b0 = np.asfortranarray(synth.dense_from_scales(7, w.p_sizes, bcs[:1], w.b_scale[:, :1]))
+    nk, nc = b0.shape
+    usz = lambda n: [bs] * (n // bs) + ([n % bs] if n % bs else [])
+    rows = 8 * bs
+    while True:
+        a = np.asfortranarray(X[:rows])
+        c = np.zeros((rows, nc), order="F")
+        sec = np.zeros(1)
+        R.ref_matmult(oracle._ptr(a), rows, nk, w.thresh, oracle._ptr(b0), nk, nc, w.thresh, oracle._ptr(c), rows, nc,
+                      0.0, 1, bs, 0, 0, w.thresh, 1.0, 0.0, None, None, oracle._ptr(sec))
+        dt = float(sec[0])
+        if dt > budget_s / 3 or rows >= X.shape[0]:
+            for _ in range(2):  # best of three at the final size
+                R.ref_matmult(oracle._ptr(a), rows, nk, w.thresh, oracle._ptr(b0), nk, nc, w.thresh, oracle._ptr(c),
+                              rows, nc, 0.0, 1, bs, 0, 0, w.thresh, 1.0, 0.0, None, None, oracle._ptr(sec))
+                dt = min(dt, float(sec[0]))
+            break
+        rows = min(X.shape[0], max(rows + bs, int(rows * min(4.0, budget_s / 3 / max(dt, 1e-3)))) // bs * bs)
+    # uniform blocks as the reference forms them (ragged last blocks screened
+    # at the full block size)
+    dA = sa.BSM.uniform(ctx, rows, nk, bs, bs, w.thresh).from_dense(a)
+    dB = sa.BSM.uniform(ctx, nk, nc, bs, bs, w.thresh).from_dense(b0)
+    dC = sa.BSM.uniform(ctx, rows, nc, bs, bs, 0.0)
+    sa.matmult(dC, dA, 0, dB, 0, w.thresh, 1.0, 0.0)
+    fl = tasks_flops(ctx.tasklist(), usz(rows), usz(nk), usz(nc))
+    diff = rel(dC.to_dense(), c)
+    del dA, dB, dC
+    return dt, fl, rows, bs, cores, diff
 
 
 def bench_rik(args):
@@ -247,6 +311,22 @@ def bench_rik(args):
                                          f"{dt:.1f} s; full step estimated from the step's algorithmic flops",
                                "est_step_s": round(est, 1),
                                "gpu_speedup_split_precision": round(est / (msmix * 1e-3), 1)}
+        log("CPU reference sample (oracle/_ref, OpenMP + MKL) ...")
+        rr = cpu_reference_rik(ctx, w, X)
+        if rr is not None:
+            dt, fl, rows, ubs, cores, diff = rr
+            rate = fl / dt
+            est = (f_tr + f_k) / rate
+            rec["cpu_reference"] = {
+                "value": round(dense_fl / est / 1e12, 6), "unit": "dense-equivalent TFLOP/s", "cores": cores,
+                "kind": "reference",
+                "sample": f"the reference's matmult (MKL dgemm per block, OpenMP on {cores} threads) on rows "
+                          f"0..{rows - 1} of X against B's first column block in uniform {ubs}-blocks (the "
+                          f"reference takes uniform blocks only): {fl:.3g} algorithmic flop in {dt:.2f} s; full "
+                          f"step estimated from the step's algorithmic flops",
+                "rel_diff_gpu_vs_reference_on_sample": diff,
+                "est_step_s": round(est, 2),
+                "gpu_speedup_split_precision": round(est / (msmix * 1e-3), 1)}
     print(json.dumps(rec))
 
 
@@ -273,7 +353,7 @@ def bench_occ(args):
     n_bf, n_aux, n_occ, bs, thresh = 600, 1200, 60, 12, 1e-10
     n_vec2, ints, mo, keys = occ_problem(n_bf, n_aux, n_occ)
     ctx = sa.Context(0)
-    teams = 8
+    teams = args.teams
     run = lambda: sa.occ_ri_k(ctx, n_bf, n_aux, n_occ, n_vec2, ints, mo, keys, bs, thresh, teams=teams)
     for _ in range(args.warmup):
         K = run()
@@ -287,7 +367,7 @@ def bench_occ(args):
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
            "data": "synthetic (seeded compressed pairs and MO coefficients, host arrays)",
            "config": {"workload": "occ_ri_k_device", "n_bf": n_bf, "n_aux": n_aux, "n_occ": n_occ, "n_vec2": n_vec2,
-                      "bs": bs, "thresh": thresh, "teams": teams, "parallelism": "single GPU, 8 streams",
+                      "bs": bs, "thresh": thresh, "teams": teams, "parallelism": f"single GPU, {teams} streams",
                       "timing": "wall clock of the call, including the upload of the integrals"}}
     if not args.no_cpu:
         import oracle
@@ -308,7 +388,57 @@ def bench_occ(args):
                                "kind": "port", "sample": f"oracle P loop over the first {g} of {n_aux} aux functions "
                                                          f"({dt:.1f} s), scaled by aux count",
                                "rel_diff_device_vs_oracle_on_sample": rel(K_dev, K_or)}
+        log("CPU reference (oracle/_ref team) ...")
+        rr = cpu_reference_occ(n_bf, n_aux, n_occ, n_vec2, ints, mo, keys, bs, thresh)
+        if rr is not None:
+            dt, cores, K_ref = rr
+            rec["cpu_reference"] = {
+                "value": round(fl / dt / 1e12, 6), "unit": "dense-equivalent TFLOP/s", "cores": cores,
+                "kind": "reference",
+                "sample": f"the whole problem: the reference's per-thread chain (occ_ri_k_step.cpp:90-120 with "
+                          f"MKL) on a team of {cores} threads, aux functions split statically, in {dt:.2f} s",
+                "rel_diff_device_vs_reference": rel(K, K_ref),
+                "gpu_speedup": round(dt / (ms * 1e-3), 2)}
     print(json.dumps(rec))
+
+
+def cpu_reference_occ(n_bf, n_aux, n_occ, n_vec2, ints, mo, keys, bs, thresh):
+    """The reference's occupied RI-K team (occ_ri_k_step.cpp:83-131): each of
+    ref_cores() host threads runs the reference's own chain (oracle/_ref) over
+    its static share of the aux functions, K_sub summed.  Returns (seconds,
+    threads, K) or None."""
+    import threading
+
+    import oracle
+
+    R = load_ref()
+    if R is None:
+        return None
+    T = min(ref_cores(), n_aux)
+    q, r = divmod(n_aux, T)
+    parts = []
+    for t in range(T):
+        p0, n = t * q + min(t, r), q + (1 if t < r else 0)
+        parts.append((np.asfortranarray(ints[:, p0:p0 + n]), n, np.zeros((n_bf, n_occ), order="F")))
+    mo_f = np.asfortranarray(mo)
+    keys_c = np.ascontiguousarray(keys, dtype=np.uint64)
+
+    def member(x, n, Kt):
+        R.ref_set_threads(1)  # nested regions serial, as inside the reference's team
+        R.ref_occ_ri_k(n_bf, n, n_occ, n_vec2, oracle._ptr(x), oracle._ptr(mo_f), oracle._ptr(keys_c), bs, thresh,
+                       oracle._ptr(Kt))
+
+    th = [threading.Thread(target=member, args=pt) for pt in parts]
+    t0 = time.perf_counter()
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    dt = time.perf_counter() - t0
+    K = np.zeros((n_bf, n_occ), order="F")
+    for _, _, Kt in parts:
+        K = K + Kt
+    return dt, T, K
 
 
 def bench_hybrid(args):
@@ -468,6 +598,7 @@ def main():
                          "(kind fp16|bf16|int8, suffix s for shared exponents in the transform, e.g. int8s; "
                          "tiles = split-precision kernel table ids, default chosen)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--teams", type=int, default=8, help="occ: team members (streams, host threads)")
     ap.add_argument("--n-occ", type=int, default=200, help="RI-K occupied orbitals (200 = the headline workload)")
     ap.add_argument("--no-check", action="store_true", help="hybrid: skip the learned-vs-heuristic comparison")
     args = ap.parse_args()
