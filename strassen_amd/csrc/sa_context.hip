@@ -99,7 +99,8 @@ constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanThreads * kScanItems;
 
-__global__ void __launch_bounds__(kScanThreads) k_scan_tiles(int64_t* __restrict__ d, int64_t n,
+// src may be d (in place); a separate src leaves the input untouched
+__global__ void __launch_bounds__(kScanThreads) k_scan_tiles(const int64_t* src, int64_t* d, int64_t n,
                                                               int64_t* __restrict__ tile_sums) {
   __shared__ int64_t wsum[kScanThreads / 64];
   const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
@@ -107,7 +108,7 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_tiles(int64_t* __restrict
   int64_t run = 0;
 #pragma unroll
   for (int i = 0; i < kScanItems; ++i) {
-    int64_t t = (base + i < n) ? d[base + i] : 0;
+    int64_t t = (base + i < n) ? src[base + i] : 0;
     v[i] = run;
     run += t;
   }
@@ -134,20 +135,18 @@ __global__ void k_scan_add(int64_t* __restrict__ d, int64_t n, const int64_t* __
   if (i < n) d[i] += tile_off[i / kScanTile];
 }
 
-static void scan_rec(sa_context* c, int64_t* d, int64_t n, int64_t* ws, int64_t* total_slot) {
+// one tile: its sum is the total, written straight to total_slot
+static void scan_rec(sa_context* c, const int64_t* src, int64_t* d, int64_t n, int64_t* ws, int64_t* total_slot) {
   const int64_t ntiles = ceil_div(n, kScanTile);
-  int64_t* part = ws;
-  hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)ntiles), dim3(kScanThreads), 0, c->stream, d, n, part);
-  if (ntiles == 1) {
-    SA_CHECK_HIP(hipMemcpyAsync(total_slot, part, sizeof(int64_t), hipMemcpyDeviceToDevice, c->stream));
-    return;
-  }
-  scan_rec(c, part, ntiles, ws + ntiles, total_slot);
+  int64_t* part = ntiles == 1 ? total_slot : ws;
+  hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)ntiles), dim3(kScanThreads), 0, c->stream, src, d, n, part);
+  if (ntiles == 1) return;
+  scan_rec(c, part, part, ntiles, ws + ntiles, total_slot);
   hipLaunchKernelGGL(k_scan_add, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, c->stream, d, n,
                      (const int64_t*)part);
 }
 
-void scan_exclusive_i64_async(sa_context* c, int64_t* d, int64_t n, int64_t* total_dev) {
+void scan_exclusive_i64_async(sa_context* c, const int64_t* src, int64_t* d, int64_t n, int64_t* total_dev) {
   int64_t need = 2, m = n;
   while (m > 1) {
     m = ceil_div(m, kScanTile);
@@ -158,9 +157,13 @@ void scan_exclusive_i64_async(sa_context* c, int64_t* d, int64_t n, int64_t* tot
   if (n == 0) {
     fill_i64(c, total_dev, 1, 0);
   } else {
-    scan_rec(c, d, n, ws, total_dev);
+    scan_rec(c, src, d, n, ws, total_dev);
   }
   SA_CHECK_HIP(hipGetLastError());
+}
+
+void scan_exclusive_i64_async(sa_context* c, int64_t* d, int64_t n, int64_t* total_dev) {
+  scan_exclusive_i64_async(c, d, d, n, total_dev);
 }
 
 int64_t scan_exclusive_i64(sa_context* c, int64_t* d, int64_t n, bool read_total) {
@@ -176,7 +179,7 @@ int64_t scan_exclusive_i64(sa_context* c, int64_t* d, int64_t n, bool read_total
   if (n == 0) {
     fill_i64(c, total_slot, 1, 0);
   } else {
-    scan_rec(c, d, n, ws, total_slot);
+    scan_rec(c, d, d, n, ws, total_slot);
   }
   SA_CHECK_HIP(hipGetLastError());
   if (!read_total) return 0;

@@ -215,6 +215,8 @@ struct ProfScope {
 int64_t scan_exclusive_i64(sa_context* c, int64_t* d, int64_t n, bool read_total = true);
 // same, total written to a device slot, no synchronisation
 void scan_exclusive_i64_async(sa_context* c, int64_t* d, int64_t n, int64_t* total_dev);
+// the same from a separate input array (left untouched) into d
+void scan_exclusive_i64_async(sa_context* c, const int64_t* src, int64_t* d, int64_t n, int64_t* total_dev);
 // read one int64 from device (sync)
 int64_t read_i64(sa_context* c, const int64_t* d);
 

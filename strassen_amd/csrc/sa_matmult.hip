@@ -291,12 +291,11 @@ ScreenResult screen_and_allocate(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool 
     hipLaunchKernelGGL(k_screen<1>, dim3((unsigned)ceil_div(nc, 4)), dim3(256), 0, c->stream, s, cnt,
                        (int32_t*)nullptr);
   }
-  SA_CHECK_HIP(hipMemcpyAsync(seg, cnt, sizeof(int64_t) * nc, hipMemcpyDeviceToDevice, c->stream));
   fill_i64(c, outf, nslots, 0);
   hipLaunchKernelGGL(k_need_and_flags, dim3((unsigned)ceil_div(nc, 256)), dim3(256), 0, c->stream,
                      (const int64_t*)cnt, (const int64_t*)C->d_off, (const int32_t*)C->d_rsz,
                      (const int32_t*)C->d_csz, nib, nc, need, outf);
-  scan_exclusive_i64_async(c, seg, nc, tot + 0);
+  scan_exclusive_i64_async(c, cnt, seg, nc, tot + 0);  // seg = exclusive scan of cnt
   scan_exclusive_i64_async(c, need, nc, tot + 1);
   scan_exclusive_i64_async(c, outf, nslots, tot + 2);
   int64_t* h = static_cast<int64_t*>(pinned(c, sizeof(int64_t) * 3));
@@ -445,6 +444,9 @@ KWindows plan_kwindows(sa_context* c, sa_bsm* A, bool ta, sa_bsm* B, bool tb, co
   const int64_t cache = (int64_t)256 << 20;
   const int32_t nkb = R.nkb;
   if (budget <= 0 || nkb < 2 || R.nouts == 0) return W;
+  // operands whose arenas hold at most two budgets in all need no byte count
+  auto arena_bytes = [](const sa_bsm* X) { return 8 * (X->base ? X->base->used : X->used); };
+  if (arena_bytes(A) + arena_bytes(B) <= 2 * budget) return W;
   DevBuf& bb = c->buf("kw_bytes");
   bb.reserve(sizeof(unsigned long long) * (size_t)(nkb + 1));
   unsigned long long* kbytes = bb.as<unsigned long long>();

@@ -147,6 +147,17 @@ def test_shared_exponents(slices, stype, ta, tb):
     the products of an output chain in one accumulation (many short products
     of ragged blocks with decaying norms, so the shared scale is not every
     block's own; vectors that are zero in one block)."""
+    shared_case(slices, stype, ta, tb, -1)
+
+
+@pytest.mark.parametrize("tile", [23, 26, 29, 31, 32, 33])
+def test_shared_exponents_every_int8_kernel(tile):
+    """The same with each 5-slice INT8 kernel forced: 64x64 and 32x32 chain
+    the products, the deep and wide kernels fold each one."""
+    shared_case(5, sa.SLICE_INT8, 0, 0, tile)
+
+
+def shared_case(slices, stype, ta, tb, tile):
     rng = np.random.default_rng(40 + slices + 3 * ta + 5 * tb)
     rsz, ksz, csz = [24, 40, 33], [17, 64, 9, 48, 30, 21, 64, 5], [36, 28]
     ni, nk, nj = sum(rsz), sum(ksz), sum(csz)
@@ -161,7 +172,7 @@ def test_shared_exponents(slices, stype, ta, tb):
     oB, dB = pair_blocked(np.asfortranarray(b_[0]), b_[1], b_[2], 0.0)
     oC, dC = empty_pair_blocked(rsz, csz)
     ref, got = run(oA, dA, ta, oB, dB, tb, oC, dC, 0.0,
-                   sa.default_opts(mode=sa.MODE_OZAKI, slices=slices, slice_type=stype, shared_exp=1))
+                   sa.default_opts(mode=sa.MODE_OZAKI, slices=slices, slice_type=stype, shared_exp=1, tile=tile))
     assert rel_fro(got, ref) < TOL
 
 
