@@ -99,11 +99,11 @@ constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanThreads * kScanItems;
 
-// src may be d (in place); a separate src leaves the input untouched
-__global__ void __launch_bounds__(kScanThreads) k_scan_tiles(const int64_t* src, int64_t* d, int64_t n,
-                                                              int64_t* __restrict__ tile_sums) {
+// Exclusive scan of tile `tile` of src into d (src may be d: each thread
+// reads its items before writing them); *sum = the tile's total.
+__device__ inline void scan_tile(const int64_t* src, int64_t* d, int64_t n, int64_t tile, int64_t* sum) {
   __shared__ int64_t wsum[kScanThreads / 64];
-  const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+  const int64_t base = tile * kScanTile + (int64_t)threadIdx.x * kScanItems;
   int64_t v[kScanItems];
   int64_t run = 0;
 #pragma unroll
@@ -127,7 +127,24 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_tiles(const int64_t* src,
 #pragma unroll
   for (int i = 0; i < kScanItems; ++i)
     if (base + i < n) d[base + i] = v[i] + excl;
-  if (threadIdx.x == kScanThreads - 1) tile_sums[blockIdx.x] = wpre + x;
+  if (threadIdx.x == kScanThreads - 1) *sum = wpre + x;
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan_tiles(const int64_t* src, int64_t* d, int64_t n,
+                                                              int64_t* __restrict__ tile_sums) {
+  scan_tile(src, d, n, blockIdx.x, tile_sums + blockIdx.x);
+}
+
+// up to 3 independent one-tile scans, one workgroup each
+struct ScanJobs {
+  const int64_t* src[3];
+  int64_t* d[3];
+  int64_t n[3];
+  int64_t* total[3];
+};
+__global__ void __launch_bounds__(kScanThreads) k_scan_jobs(ScanJobs j) {
+  const int b = blockIdx.x;
+  scan_tile(j.src[b], j.d[b], j.n[b], 0, j.total[b]);
 }
 
 __global__ void k_scan_add(int64_t* __restrict__ d, int64_t n, const int64_t* __restrict__ tile_off) {
@@ -164,6 +181,25 @@ void scan_exclusive_i64_async(sa_context* c, const int64_t* src, int64_t* d, int
 
 void scan_exclusive_i64_async(sa_context* c, int64_t* d, int64_t n, int64_t* total_dev) {
   scan_exclusive_i64_async(c, d, d, n, total_dev);
+}
+
+void scan3_exclusive_i64_async(sa_context* c, const int64_t* const src[3], int64_t* const d[3], const int64_t n[3],
+                               int64_t* const total[3]) {
+  bool small = true;
+  for (int i = 0; i < 3; ++i) small = small && n[i] >= 1 && n[i] <= kScanTile;
+  if (!small) {
+    for (int i = 0; i < 3; ++i) scan_exclusive_i64_async(c, src[i], d[i], n[i], total[i]);
+    return;
+  }
+  ScanJobs j;
+  for (int i = 0; i < 3; ++i) {
+    j.src[i] = src[i];
+    j.d[i] = d[i];
+    j.n[i] = n[i];
+    j.total[i] = total[i];
+  }
+  hipLaunchKernelGGL(k_scan_jobs, dim3(3), dim3(kScanThreads), 0, c->stream, j);
+  SA_CHECK_HIP(hipGetLastError());
 }
 
 int64_t scan_exclusive_i64(sa_context* c, int64_t* d, int64_t n, bool read_total) {

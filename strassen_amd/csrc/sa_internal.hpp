@@ -217,6 +217,10 @@ int64_t scan_exclusive_i64(sa_context* c, int64_t* d, int64_t n, bool read_total
 void scan_exclusive_i64_async(sa_context* c, int64_t* d, int64_t n, int64_t* total_dev);
 // the same from a separate input array (left untouched) into d
 void scan_exclusive_i64_async(sa_context* c, const int64_t* src, int64_t* d, int64_t n, int64_t* total_dev);
+// three independent scans (src[i] -> d[i], n[i] values, total[i]); one launch
+// when each fits one scan tile (2048 values)
+void scan3_exclusive_i64_async(sa_context* c, const int64_t* const src[3], int64_t* const d[3], const int64_t n[3],
+                               int64_t* const total[3]);
 // read one int64 from device (sync)
 int64_t read_i64(sa_context* c, const int64_t* d);
 

@@ -100,6 +100,7 @@ __global__ void __launch_bounds__(256) k_screen(ScreenArgs s, int64_t* __restric
 __global__ void k_row_count(const int64_t* __restrict__ aoff, int32_t nrows, int32_t nk, int64_t sr, int64_t sk,
                             int64_t* __restrict__ cnt) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r == 0) cnt[nrows] = 0;  // the extra slot: its exclusive scan is the total
   if (r >= nrows) return;
   int64_t n = 0;
   for (int32_t k = 0; k < nk; ++k) n += aoff[r * sr + k * sk] >= 0 ? 1 : 0;
@@ -180,15 +181,25 @@ __global__ void k_build_records(MMTables t, GemmOut* __restrict__ outs, GemmProd
   }
 }
 
+// one thread per output slot (out_slot order, padded to whole groups): the
+// slot's flag, 0 for padding, and for a real C block its arena need
 __global__ void k_need_and_flags(const int64_t* __restrict__ cnt, const int64_t* __restrict__ coff,
                                  const int32_t* __restrict__ crsz, const int32_t* __restrict__ ccsz, int32_t nib,
-                                 int64_t nc, int64_t* __restrict__ need, int64_t* __restrict__ outflag) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nc) return;
+                                 int32_t njb, int64_t nslots, int64_t* __restrict__ need,
+                                 int64_t* __restrict__ outflag) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nslots) return;
+  const int64_t ngi = ceil_div(nib, kOutGroup), g = s / (kOutGroup * kOutGroup);
+  const int w = (int)(s % (kOutGroup * kOutGroup));
+  const int ib = (int)(g % ngi) * kOutGroup + w % kOutGroup, jb = (int)(g / ngi) * kOutGroup + w / kOutGroup;
+  if (ib >= nib || jb >= njb) {
+    outflag[s] = 0;
+    return;
+  }
+  const int64_t c = (int64_t)ib + (int64_t)jb * nib;
   const bool any = cnt[c] > 0;
-  const int ib = (int)(c % nib), jb = (int)(c / nib);
   need[c] = (any && coff[c] < 0) ? (int64_t)crsz[ib] * ccsz[jb] : 0;
-  outflag[out_slot(ib, jb, nib)] = any ? 1 : 0;
+  outflag[s] = any ? 1 : 0;
 }
 
 // geometry of op(X) for screening / checking
@@ -265,7 +276,6 @@ ScreenResult screen_and_allocate(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool 
   {
     const int64_t sr = ta ? A->nrb : 1, sk = ta ? 1 : A->nrb;
     int64_t* rc = brc.as<int64_t>();
-    fill_i64(c, rc + nib, 1, 0);
     hipLaunchKernelGGL(k_row_count, dim3((unsigned)ceil_div(nib, 256)), dim3(256), 0, c->stream,
                        (const int64_t*)A->d_off, nib, nkb, sr, sk, rc);
     scan_exclusive_i64_async(c, rc, nib + 1, tot + 3);
@@ -291,13 +301,16 @@ ScreenResult screen_and_allocate(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool 
     hipLaunchKernelGGL(k_screen<1>, dim3((unsigned)ceil_div(nc, 4)), dim3(256), 0, c->stream, s, cnt,
                        (int32_t*)nullptr);
   }
-  fill_i64(c, outf, nslots, 0);
-  hipLaunchKernelGGL(k_need_and_flags, dim3((unsigned)ceil_div(nc, 256)), dim3(256), 0, c->stream,
+  hipLaunchKernelGGL(k_need_and_flags, dim3((unsigned)ceil_div(nslots, 256)), dim3(256), 0, c->stream,
                      (const int64_t*)cnt, (const int64_t*)C->d_off, (const int32_t*)C->d_rsz,
-                     (const int32_t*)C->d_csz, nib, nc, need, outf);
-  scan_exclusive_i64_async(c, cnt, seg, nc, tot + 0);  // seg = exclusive scan of cnt
-  scan_exclusive_i64_async(c, need, nc, tot + 1);
-  scan_exclusive_i64_async(c, outf, nslots, tot + 2);
+                     (const int32_t*)C->d_csz, nib, njb, nslots, need, outf);
+  {  // seg = exclusive scan of cnt; need and outf in place
+    const int64_t* src[3] = {cnt, need, outf};
+    int64_t* const dst[3] = {seg, need, outf};
+    const int64_t n[3] = {nc, nc, nslots};
+    int64_t* const totals[3] = {tot + 0, tot + 1, tot + 2};
+    scan3_exclusive_i64_async(c, src, dst, n, totals);
+  }
   int64_t* h = static_cast<int64_t*>(pinned(c, sizeof(int64_t) * 3));
   SA_CHECK_HIP(hipMemcpyAsync(h, tot, sizeof(int64_t) * 3, hipMemcpyDeviceToHost, c->stream));
   SA_CHECK_HIP(hipStreamSynchronize(c->stream));
