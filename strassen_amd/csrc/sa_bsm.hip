@@ -370,13 +370,12 @@ void bsm_ensure_capacity(sa_bsm* m, int64_t elems) {
   m->arena.reserve_keep(want, m->ctx->stream);
 }
 
-void launch_block_norms(sa_bsm* m) {
+void launch_block_norms(sa_bsm* m, const int64_t* mask) {
   const int64_t nb = m->nb();
   if (nb == 0) return;
   ProfScope ps(m->ctx, "norms");
   hipLaunchKernelGGL(k_norms_arena, dim3((unsigned)ceil_div(nb, 4)), dim3(256), 0, m->ctx->stream,
-                     (const double*)m->data(), (const int64_t*)m->d_off, geo(m), nb, (const int64_t*)nullptr,
-                     m->d_norm);
+                     (const double*)m->data(), (const int64_t*)m->d_off, geo(m), nb, mask, m->d_norm);
 }
 
 static sa_bsm* make_bsm(sa_context* c, const std::vector<int32_t>& rsz, const std::vector<int32_t>& rscr,

@@ -236,7 +236,8 @@ void bsm_upload_geometry(sa_bsm* m);
 void bsm_alloc_tables(sa_bsm* m);
 
 // norm kernel launcher: recompute norms of every present block (or all if mask==nullptr)
-void launch_block_norms(sa_bsm* m);
+// canonical block norms; mask (may be null) selects the blocks to refresh
+void launch_block_norms(sa_bsm* m, const int64_t* mask = nullptr);
 
 // ------------------------------------------------------------ grouped GEMM --
 // One output block (column-major, ld = m) and its segment [p0, p1) of products

@@ -96,27 +96,38 @@ __global__ void __launch_bounds__(256) k_screen(ScreenArgs s, int64_t* __restric
 }
 
 // CSR of the present blocks of op(A) by row: row ib, kb ascending.
-// A block (ib, kb) of op(A) sits at table index ib*sr + kb*sk.
-__global__ void k_row_count(const int64_t* __restrict__ aoff, int32_t nrows, int32_t nk, int64_t sr, int64_t sk,
-                            int64_t* __restrict__ cnt) {
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r == 0) cnt[nrows] = 0;  // the extra slot: its exclusive scan is the total
+// A block (ib, kb) of op(A) sits at table index ib*sr + kb*sk.  One wave per
+// row (4 per workgroup), 64 kb per step.
+__global__ void __launch_bounds__(256) k_row_count(const int64_t* __restrict__ aoff, int32_t nrows, int32_t nk,
+                                                   int64_t sr, int64_t sk, int64_t* __restrict__ cnt) {
+  const int lane = threadIdx.x & 63;
+  const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (r == 0 && lane == 0) cnt[nrows] = 0;  // the extra slot: its exclusive scan is the total
   if (r >= nrows) return;
   int64_t n = 0;
-  for (int32_t k = 0; k < nk; ++k) n += aoff[r * sr + k * sk] >= 0 ? 1 : 0;
-  cnt[r] = n;
+  for (int32_t k0 = 0; k0 < nk; k0 += 64) {
+    const int32_t k = k0 + lane;
+    n += __popcll(__ballot(k < nk && aoff[r * sr + (int64_t)k * sk] >= 0));
+  }
+  if (lane == 0) cnt[r] = n;
 }
 
-__global__ void k_row_fill(const int64_t* __restrict__ aoff, int32_t nrows, int32_t nk, int64_t sr, int64_t sk,
-                           const int64_t* __restrict__ start, int32_t* __restrict__ rowstart,
-                           int32_t* __restrict__ list) {
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_row_fill(const int64_t* __restrict__ aoff, int32_t nrows, int32_t nk,
+                                                  int64_t sr, int64_t sk, const int64_t* __restrict__ start,
+                                                  int32_t* __restrict__ rowstart, int32_t* __restrict__ list) {
+  const int lane = threadIdx.x & 63;
+  const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (r > nrows) return;
   int64_t s = start[r];
-  rowstart[r] = (int32_t)s;
+  if (lane == 0) rowstart[r] = (int32_t)s;
   if (r == nrows) return;
-  for (int32_t k = 0; k < nk; ++k)
-    if (aoff[r * sr + k * sk] >= 0) list[s++] = k;
+  for (int32_t k0 = 0; k0 < nk; k0 += 64) {
+    const int32_t k = k0 + lane;
+    const bool present = k < nk && aoff[r * sr + (int64_t)k * sk] >= 0;
+    const uint64_t mask = __ballot(present);
+    if (present) list[s + __popcll(mask & ((1ull << lane) - 1ull))] = k;
+    s += __popcll(mask);
+  }
 }
 
 struct MMTables {
@@ -276,10 +287,10 @@ ScreenResult screen_and_allocate(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool 
   {
     const int64_t sr = ta ? A->nrb : 1, sk = ta ? 1 : A->nrb;
     int64_t* rc = brc.as<int64_t>();
-    hipLaunchKernelGGL(k_row_count, dim3((unsigned)ceil_div(nib, 256)), dim3(256), 0, c->stream,
+    hipLaunchKernelGGL(k_row_count, dim3((unsigned)ceil_div(nib, 4)), dim3(256), 0, c->stream,
                        (const int64_t*)A->d_off, nib, nkb, sr, sk, rc);
     scan_exclusive_i64_async(c, rc, nib + 1, tot + 3);
-    hipLaunchKernelGGL(k_row_fill, dim3((unsigned)ceil_div(nib + 1, 256)), dim3(256), 0, c->stream,
+    hipLaunchKernelGGL(k_row_fill, dim3((unsigned)ceil_div(nib + 1, 4)), dim3(256), 0, c->stream,
                        (const int64_t*)A->d_off, nib, nkb, sr, sk, (const int64_t*)rc, brs.as<int32_t>(),
                        brl.as<int32_t>());
   }

@@ -13,6 +13,8 @@
 //    screened multiply over a reshape view of B' whose rows are (P, i) pairs.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <cstring>
 #include <string>
 #include <thread>
 #include <vector>
@@ -147,27 +149,71 @@ static void inv_sqrt_device(sa_context* c, int64_t n, const double* dM, double* 
 // zero for every P, as in the reference's reused dense buffer), then each
 // touched block gets the canonical norm and the copy-in drop rule
 // (blocksparsematrix.hpp:55-59) of ints_3c_decompressed_bsm.copy_from_input_matrix.
-__global__ void k_occ_scatter(const double* __restrict__ vals, const int64_t* __restrict__ dest, int64_t n,
-                              double* __restrict__ arena) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int64_t d = dest[i];
-  if (d >= 0) arena[d] = vals[i];
+// Host -> device copy of a caller's (pageable) array through two pinned 8 MB
+// halves of the context's staging buffer: the host fills one half while the
+// other is in flight (stream-ordered; a half is refilled only after the copy
+// that read it has completed; later users of the staging buffer on this
+// stream are ordered behind the last copies).
+static void upload_staged(sa_context* x, void* dst, const void* src, size_t bytes) {
+  constexpr size_t kChunk = (size_t)8 << 20;
+  char* stage = static_cast<char*>(pinned(x, 2 * kChunk));
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  SA_CHECK_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+  SA_CHECK_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+  try {
+    size_t i = 0;
+    for (size_t off = 0; off < bytes; off += kChunk, ++i) {
+      const size_t n = std::min(kChunk, bytes - off);
+      char* half = stage + (i & 1) * kChunk;
+      if (i >= 2) SA_CHECK_HIP(hipEventSynchronize(ev[i & 1]));
+      std::memcpy(half, static_cast<const char*>(src) + off, n);
+      SA_CHECK_HIP(hipMemcpyAsync(static_cast<char*>(dst) + off, half, n, hipMemcpyHostToDevice, x->stream));
+      SA_CHECK_HIP(hipEventRecord(ev[i & 1], x->stream));
+    }
+  } catch (...) {
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    throw;
+  }
+  (void)hipEventDestroy(ev[0]);
+  (void)hipEventDestroy(ev[1]);
 }
 
-__global__ void __launch_bounds__(256) k_occ_block_norms(const double* __restrict__ arena,
-                                                         const int64_t* __restrict__ slot,
-                                                         const int32_t* __restrict__ rsz,
-                                                         const int32_t* __restrict__ csz,
-                                                         const int32_t* __restrict__ rscr,
-                                                         const int32_t* __restrict__ cscr, int32_t nrb, int64_t nb,
-                                                         double thresh, int64_t* __restrict__ off,
-                                                         double* __restrict__ norm) {
+// aux functions per batch of the occupied RI-K: SA_OCC_BATCH when set (> 0),
+// else as many as 256 MB of stacked U hold
+static int64_t occ_batch_cap() {
+  const char* v = std::getenv("SA_OCC_BATCH");
+  return v ? std::atoll(v) : 0;
+}
+
+// Batched occupied RI-K: a batch of nP aux functions of one team member runs
+// as three screened multiplies over stacked matrices (U_P side by side, mui_P
+// side by side, ijs_P stacked), each P's blocks in its own segment.
+
+// U_P values of the batch into segment P of the stacked arena (P-major input)
+__global__ void k_occ_scatter_batch(const double* __restrict__ vals, const int64_t* __restrict__ dest, int64_t n_vec2,
+                                    int64_t nP, int64_t total, double* __restrict__ arena) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_vec2 * nP) return;
+  const int64_t P = i / n_vec2, d = dest[i % n_vec2];
+  if (d >= 0) arena[P * total + d] = vals[i];
+}
+
+// block table and norms of the stacked U (segments P >= nP stay empty)
+__global__ void __launch_bounds__(256) k_occ_block_norms_batch(const double* __restrict__ arena,
+                                                               const int64_t* __restrict__ slot,
+                                                               const int32_t* __restrict__ rsz,
+                                                               const int32_t* __restrict__ csz,
+                                                               const int32_t* __restrict__ rscr,
+                                                               const int32_t* __restrict__ cscr, int32_t nrb,
+                                                               int64_t nb, int64_t nseg, int64_t nP, int64_t total,
+                                                               double thresh, int64_t* __restrict__ off,
+                                                               double* __restrict__ norm) {
   const int lane = threadIdx.x & 63;
   const int64_t b = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (b >= nb) return;
-  const int64_t sl = slot[b];
-  if (sl < 0) {  // all-zero block: dropped by the copy-in rule (threshold > 0)
+  if (b >= nb * nseg) return;
+  const int64_t P = b / nb, sl = P < nP ? slot[b % nb] : -1;
+  if (sl < 0) {  // all-zero block (or unused segment): dropped by the copy-in rule
     if (lane == 0) {
       off[b] = -1;
       norm[b] = 0.0;
@@ -175,11 +221,69 @@ __global__ void __launch_bounds__(256) k_occ_block_norms(const double* __restric
     return;
   }
   const int rb = (int)(b % nrb), cb = (int)(b / nrb);
-  const double est = wave_block_norm(arena + sl, (int64_t)rsz[rb] * csz[cb], lane);
+  const double est = wave_block_norm(arena + P * total + sl, (int64_t)rsz[rb] * csz[cb], lane);
   if (lane == 0) {
     const bool drop = est < thresh * (double)((int64_t)cscr[cb] * (int64_t)rscr[rb]);
-    off[b] = drop ? -1 : sl;
+    off[b] = drop ? -1 : P * total + sl;
     norm[b] = drop ? 0.0 : est;
+  }
+}
+
+// block-diagonal repeat of C's block table (all segments share C's arena copy)
+__global__ void k_occ_blockdiag(const int64_t* __restrict__ coff, const double* __restrict__ cnorm, int32_t nrb,
+                                int32_t nocb, int32_t nseg, int64_t* __restrict__ off, double* __restrict__ norm) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t R = (int64_t)nrb * nseg;
+  if (b >= R * nocb * nseg) return;
+  const int64_t row = b % R, col = b / R;
+  const int64_t P = row / nrb, Q = col / nocb;
+  const int64_t src = row % nrb + (col % nocb) * nrb;
+  off[b] = P == Q ? coff[src] : -1;
+  norm[b] = P == Q ? cnorm[src] : 0.0;
+}
+
+// ijs_P = ij_P + ij_P^T as occ_ri_k_step.cpp:109-110 does it (ij_sym = ij;
+// ij_sym.add_transpose(ij)): ij is n_occ x (nseg n_occ) with segment P = ij_P,
+// ijs is (nseg n_occ) x n_occ with segment P = ijs_P.  A block exists where
+// ij_P has it or its transposed partner; its norm is refreshed (touched) only
+// where the partner exists, else it keeps ij_P's fresh-block norm 0
+// (blocksparsematrix.hpp:225-248).  Fixed arena slots: segment P, block (a, b)
+// at P n_occ^2 + start_b n_occ + start_a size_b.
+__global__ void __launch_bounds__(256) k_occ_sym(const double* __restrict__ ij, const int64_t* __restrict__ ij_off,
+                                                 const int32_t* __restrict__ osz, int32_t nocb, int64_t n_occ,
+                                                 int32_t nseg, double* __restrict__ out,
+                                                 int64_t* __restrict__ out_off, double* __restrict__ out_norm,
+                                                 int64_t* __restrict__ touched) {
+  const int64_t ob = blockIdx.x;  // (P, a, b), a fastest
+  const int64_t per = (int64_t)nocb * nocb;
+  const int64_t P = ob / per;
+  const int a = (int)(ob % per % nocb), b = (int)(ob % per / nocb);
+  const int64_t ia = a + (P * nocb + b) * nocb, ib = b + (P * nocb + a) * nocb;  // ij blocks (a, b), (b, a) of P
+  const int64_t o = (P * nocb + a) + (int64_t)b * ((int64_t)nseg * nocb);     // ijs block (P, a), b
+  const int64_t oa = ij_off[ia], ob2 = ij_off[ib];
+  if (oa < 0 && ob2 < 0) {
+    if (threadIdx.x == 0) {
+      out_off[o] = -1;
+      out_norm[o] = 0.0;
+      touched[o] = 0;
+    }
+    return;
+  }
+  int64_t sa = 0, sb = 0;
+  for (int i = 0; i < a; ++i) sa += osz[i];
+  for (int i = 0; i < b; ++i) sb += osz[i];
+  const int ma = osz[a], mb = osz[b];
+  const int64_t dst = P * n_occ * n_occ + sb * n_occ + sa * mb;
+  for (int e = threadIdx.x; e < ma * mb; e += blockDim.x) {
+    const int r = e % ma, c = e / ma;
+    double v = oa >= 0 ? ij[oa + e] : 0.0;
+    if (ob2 >= 0) v += ij[ob2 + c + (int64_t)r * mb];
+    out[dst + e] = v;
+  }
+  if (threadIdx.x == 0) {
+    out_off[o] = dst;
+    out_norm[o] = 0.0;
+    touched[o] = ob2 >= 0 ? 1 : 0;
   }
 }
 
@@ -358,8 +462,7 @@ int sa_occ_ri_k(sa_ctx_t c, int64_t n_bf, int64_t n_aux, int64_t n_occ, int64_t 
           return m;
         };
         // occ_ri_k_step.cpp:84-98 (U's copy-in threshold is the reference's 1e-20)
-        sa_bsm_t Cb = mk(n_bf, n_occ, 0.0), U = mk(n_bf, n_bf, 1e-20);
-        sa_bsm_t mui = mk(n_bf, n_occ, 0.0), ij = mk(n_occ, n_occ, 0.0), ijs = mk(n_occ, n_occ, 0.0);
+        sa_bsm_t Cb = mk(n_bf, n_occ, 0.0);
         ok(sa_bsm_from_dense(Cb, mo, n_bf));
         DevBuf& bints = x->buf("occ_ints");
         DevBuf& bdest = x->buf("occ_dest");
@@ -368,32 +471,84 @@ int sa_occ_ri_k(sa_ctx_t c, int64_t n_bf, int64_t n_aux, int64_t n_occ, int64_t 
         bdest.reserve(sizeof(int64_t) * (size_t)std::max<int64_t>(1, n_vec2));
         bslot.reserve(sizeof(int64_t) * (size_t)std::max<int64_t>(1, nb));
         if (n_vec2 * np)
-          SA_CHECK_HIP(hipMemcpyAsync(bints.p, ints3c + P0 * n_vec2, sizeof(double) * (size_t)(n_vec2 * np),
-                                      hipMemcpyHostToDevice, x->stream));
+          upload_staged(x, bints.p, ints3c + P0 * n_vec2, sizeof(double) * (size_t)(n_vec2 * np));
         if (n_vec2)
           SA_CHECK_HIP(hipMemcpyAsync(bdest.p, dest.data(), sizeof(int64_t) * (size_t)n_vec2, hipMemcpyHostToDevice,
                                       x->stream));
         SA_CHECK_HIP(
             hipMemcpyAsync(bslot.p, slot.data(), sizeof(int64_t) * (size_t)nb, hipMemcpyHostToDevice, x->stream));
-        bsm_ensure_capacity(U, total);
-        U->used = total;
-        fill_f64(x, U->data(), total, 0.0);
-        for (int64_t P = 0; P < np; ++P) {  // occ_ri_k_step.cpp:100-118
+        // occ_ri_k_step.cpp:100-118 over batches of nseg aux functions.  The
+        // reference empties mui and ij (beta = 0) for every P, so the block
+        // products of a batch are exactly those of its P one by one, screened
+        // by the same norms; only the FP64 order of K_sub's sum over P changes
+        // (one accumulation per batch).
+        const int64_t per_seg = std::max<int64_t>(1, total) * (int64_t)sizeof(double);
+        const int64_t cap = occ_batch_cap();
+        const int64_t nseg = std::max<int64_t>(1, std::min<int64_t>(std::max<int64_t>(np, 1),
+                                                                       std::max<int64_t>(1, cap > 0 ? cap :
+                                                                                             (256ll << 20) / per_seg)));
+        const int32_t nocb = (int32_t)ceil_div(n_occ, (int64_t)bs);
+        std::vector<int32_t> sbf((size_t)nrb), socc((size_t)nocb);
+        for (int64_t i = 0; i < nrb; ++i) sbf[(size_t)i] = (int32_t)bsize(i);
+        for (int32_t i = 0; i < nocb; ++i) socc[(size_t)i] = (int32_t)std::min<int64_t>(bs, n_occ - (int64_t)i * bs);
+        auto rep = [&](const std::vector<int32_t>& v) {
+          std::vector<int32_t> r;
+          for (int64_t P = 0; P < nseg; ++P) r.insert(r.end(), v.begin(), v.end());
+          return r;
+        };
+        const std::vector<int32_t> sbf_n = rep(sbf), socc_n = rep(socc);
+        auto full = [&](size_t n) { return std::vector<int32_t>(n, bs); };  // screening sizes: the target bs
+        auto mkb = [&](const std::vector<int32_t>& r, const std::vector<int32_t>& q, double th) {
+          sa_bsm_t m = nullptr;
+          const std::vector<int32_t> rs = full(r.size()), qs = full(q.size());
+          ok(sa_bsm_create_blocked(x, (int32_t)r.size(), r.data(), rs.data(), (int32_t)q.size(), q.data(),
+                                   qs.data(), th, &m));
+          own.push_back(m);
+          return m;
+        };
+        sa_bsm_t Ub = mkb(sbf, sbf_n, 1e-20);         // [U_1 ... U_nseg]; U's copy-in threshold
+        sa_bsm_t Cd = mkb(sbf_n, socc_n, 0.0);        // diag(C, ..., C)
+        sa_bsm_t mub = mkb(sbf, socc_n, 0.0);         // [mui_1 ... mui_nseg]
+        sa_bsm_t ijb = mkb(socc, socc_n, 0.0);        // [ij_1 ... ij_nseg]
+        sa_bsm_t ijsb = mkb(socc_n, socc, 0.0);       // [ijs_1; ...; ijs_nseg]
+        bsm_ensure_capacity(Ub, nseg * total);
+        Ub->used = nseg * total;
+        fill_f64(x, Ub->data(), nseg * total, 0.0);
+        bsm_ensure_capacity(Cd, std::max<int64_t>(Cb->used, 1));
+        Cd->used = Cb->used;
+        if (Cb->used)
+          SA_CHECK_HIP(hipMemcpyAsync(Cd->data(), Cb->data(), sizeof(double) * (size_t)Cb->used,
+                                      hipMemcpyDeviceToDevice, x->stream));
+        hipLaunchKernelGGL(k_occ_blockdiag, dim3((unsigned)ceil_div(Cd->nb(), 256)), dim3(256), 0, x->stream,
+                           (const int64_t*)Cb->d_off, (const double*)Cb->d_norm, (int32_t)nrb, nocb, (int32_t)nseg,
+                           Cd->d_off, Cd->d_norm);
+        bsm_ensure_capacity(ijsb, nseg * n_occ * n_occ);
+        ijsb->used = nseg * n_occ * n_occ;
+        DevBuf& btouch = x->buf("occ_touch");
+        btouch.reserve(sizeof(int64_t) * (size_t)(ijsb->nb() + 1));
+        SA_CHECK_HIP(hipGetLastError());
+        for (int64_t Pb = 0; Pb < np; Pb += nseg) {
+          const int64_t nP = std::min<int64_t>(nseg, np - Pb);
           if (n_vec2)
-            hipLaunchKernelGGL(k_occ_scatter, dim3((unsigned)ceil_div(n_vec2, 256)), dim3(256), 0, x->stream,
-                               (const double*)bints.as<double>() + P * n_vec2, (const int64_t*)bdest.as<int64_t>(),
-                               n_vec2, U->data());
-          hipLaunchKernelGGL(k_occ_block_norms, dim3((unsigned)ceil_div(nb, 4)), dim3(256), 0, x->stream,
-                             (const double*)U->data(), (const int64_t*)bslot.as<int64_t>(), (const int32_t*)U->d_rsz,
-                             (const int32_t*)U->d_csz, (const int32_t*)U->d_rscr, (const int32_t*)U->d_cscr, U->nrb,
-                             nb, U->thresh, U->d_off, U->d_norm);
+            hipLaunchKernelGGL(k_occ_scatter_batch, dim3((unsigned)ceil_div(n_vec2 * nP, 256)), dim3(256), 0,
+                               x->stream, (const double*)bints.as<double>() + Pb * n_vec2,
+                               (const int64_t*)bdest.as<int64_t>(), n_vec2, nP, total, Ub->data());
+          hipLaunchKernelGGL(k_occ_block_norms_batch, dim3((unsigned)ceil_div(nb * nseg, 4)), dim3(256), 0,
+                             x->stream, (const double*)Ub->data(), (const int64_t*)bslot.as<int64_t>(),
+                             (const int32_t*)Ub->d_rsz, (const int32_t*)Ub->d_csz, (const int32_t*)Ub->d_rscr,
+                             (const int32_t*)Ub->d_cscr, Ub->nrb, nb, nseg, nP, total, Ub->thresh, Ub->d_off,
+                             Ub->d_norm);
           SA_CHECK_HIP(hipGetLastError());
-          ok(sa_matmult(mui, U, 0, Cb, 0, thresh, 1.0, 0.0, opts));
-          ok(sa_bsm_calc_norms(mui));
-          ok(sa_matmult(ij, Cb, 1, mui, 0, thresh, 1.0, 0.0, opts));
-          ok(sa_bsm_copy(ijs, ij));
-          ok(sa_bsm_add_transpose(ijs, ij));
-          ok(sa_matmult(Ks[(size_t)t], mui, 0, ijs, 0, thresh, 1.0, 1.0, opts));
+          ok(sa_matmult(mub, Ub, 0, Cd, 0, thresh, 1.0, 0.0, opts));  // mui_P = U_P C
+          ok(sa_bsm_calc_norms(mub));
+          ok(sa_matmult(ijb, Cb, 1, mub, 0, thresh, 1.0, 0.0, opts));  // ij_P = C^T mui_P
+          hipLaunchKernelGGL(k_occ_sym, dim3((unsigned)(nseg * nocb * nocb)), dim3(256), 0, x->stream,
+                             (const double*)ijb->data(), (const int64_t*)ijb->d_off, (const int32_t*)ijb->d_rsz,
+                             nocb, n_occ, (int32_t)nseg, ijsb->data(), ijsb->d_off, ijsb->d_norm,
+                             btouch.as<int64_t>());
+          SA_CHECK_HIP(hipGetLastError());
+          launch_block_norms(ijsb, btouch.as<int64_t>());
+          ok(sa_matmult(Ks[(size_t)t], mub, 0, ijsb, 0, thresh, 1.0, 1.0, opts));  // K_sub += sum_P mui_P ijs_P
         }
         SA_CHECK_HIP(hipStreamSynchronize(x->stream));
       });

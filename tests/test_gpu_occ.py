@@ -69,6 +69,20 @@ def test_synthetic_vs_oracle(n_bf, n_aux, n_occ, bs, thresh, dup, teams):
     assert rel_fro(K, Ko) < 1e-12
 
 
+@pytest.mark.parametrize("batch", ["1", "3", "7", "0"])
+@pytest.mark.parametrize("n_bf,n_aux,n_occ,bs,thresh,teams", [(70, 40, 12, 8, 1e-10, 1), (45, 17, 9, 5, 1e-6, 3)])
+def test_aux_batches(monkeypatch, batch, n_bf, n_aux, n_occ, bs, thresh, teams):
+    """Aux functions run in batches of 1, 3, 7 or as many as fit (ragged bf and
+    occ blocks, a partial last batch): the same screened products as one P at
+    a time, K within the FP64 bar of the oracle's team."""
+    monkeypatch.setenv("SA_OCC_BATCH", batch)
+    n_vec2, ints, mo, keys = synthetic(n_bf + n_aux + 7, n_bf, n_aux, n_occ)
+    K = sa.occ_ri_k(ctx(), n_bf, n_aux, n_occ, n_vec2, ints, mo, keys, bs, thresh, teams=teams)
+    Ko = team_oracle(n_bf, n_aux, n_occ, n_vec2, ints, mo, keys, bs, thresh, teams)
+    assert np.linalg.norm(Ko) > 0
+    assert rel_fro(K, Ko) < 1e-12
+
+
 def test_split_precision(h2o):
     """The three products in split precision (5 INT8 slices): bar 1e-10."""
     n_vec2, ints, mo, keys = synthetic(7, 64, 30, 10)
