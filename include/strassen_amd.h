@@ -206,9 +206,11 @@ int sa_ri_k_step(sa_bsm_t X, sa_bsm_t B, sa_bsm_t Bp, sa_bsm_t K, int32_t n_mu, 
  * entirely on the device: K_{mu j} = sum_{P,i} (mu i|P)(ij|P).  The OpenMP
  * team of the reference becomes `teams` members (<= 0: 8), each with its own
  * stream and host thread: P is split as OpenMP's static schedule (contiguous
- * ranges, the first n_aux % teams members one more), each member reuses its
- * mui, ij, ijs and K_sub across its range, and the K_sub are added into K in
- * member order (the reference adds them in a critical section in arrival
+ * ranges, the first n_aux % teams members one more), each member runs its
+ * range in batches of aux functions (stacked U_P, mui_P, ij_P, ijs_P: the same
+ * screened block products as one P at a time, since the reference empties mui
+ * and ij for every P; environment variable SA_OCC_BATCH caps the batch), and
+ * the K_sub are added into K in member order (the reference adds them in a critical section in arrival
  * order; teams = 1 is its one-thread run).  ints3c is the compressed
  * 3-centre tensor (n_vec2 x n_aux, column-major: column P holds the pairs of
  * aux function P), keys[n] = r + n_bf c places pair n in the n_bf x n_bf
@@ -217,7 +219,8 @@ int sa_ri_k_step(sa_bsm_t X, sa_bsm_t B, sa_bsm_t Bp, sa_bsm_t K, int32_t n_mu, 
  * n_occ result (all host, column-major).  The integrals are uploaded once and
  * decompressed per P on the device straight into the block arena of U_P (no
  * dense n_bf x n_bf matrix), with U's copy-in drop rule at 1e-20 as in the
- * reference; then mui = U C (norms refreshed), ij = C^T mui, ijs = ij + ij^T,
+ * reference; then mui = U C (norms refreshed), ij = C^T mui, ijs = ij + ij^T
+ * (norms refreshed where the transposed block exists, as add_transpose does),
  * Ks += mui ijs, each screened at thresh and run in the mode of opts (NULL:
  * FP64).  Errors: SA_ERR_INVALID for a key beyond n_bf^2 (the reference's
  * assert, occ_ri_k_step.cpp:139) or bad sizes. */
