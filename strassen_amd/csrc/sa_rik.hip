@@ -482,12 +482,13 @@ int sa_occ_ri_k(sa_ctx_t c, int64_t n_bf, int64_t n_aux, int64_t n_occ, int64_t 
         // products of a batch are exactly those of its P one by one, screened
         // by the same norms; only the FP64 order of K_sub's sum over P changes
         // (one accumulation per batch).
+        const int32_t nocb = (int32_t)ceil_div(n_occ, (int64_t)bs);
         const int64_t per_seg = std::max<int64_t>(1, total) * (int64_t)sizeof(double);
         const int64_t cap = occ_batch_cap();
-        const int64_t nseg = std::max<int64_t>(1, std::min<int64_t>(std::max<int64_t>(np, 1),
-                                                                       std::max<int64_t>(1, cap > 0 ? cap :
-                                                                                             (256ll << 20) / per_seg)));
-        const int32_t nocb = (int32_t)ceil_div(n_occ, (int64_t)bs);
+        // block-diagonal C's table holds nseg^2 nrb nocb entries: at most 2^24
+        const int64_t diag_cap = (int64_t)std::sqrt((double)((int64_t)1 << 24) / (double)(nrb * (int64_t)nocb));
+        int64_t nseg = cap > 0 ? cap : std::min<int64_t>((256ll << 20) / per_seg, diag_cap);
+        nseg = std::max<int64_t>(1, std::min<int64_t>(nseg, std::max<int64_t>(np, 1)));
         std::vector<int32_t> sbf((size_t)nrb), socc((size_t)nocb);
         for (int64_t i = 0; i < nrb; ++i) sbf[(size_t)i] = (int32_t)bsize(i);
         for (int32_t i = 0; i < nocb; ++i) socc[(size_t)i] = (int32_t)std::min<int64_t>(bs, n_occ - (int64_t)i * bs);
