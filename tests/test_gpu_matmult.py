@@ -84,6 +84,24 @@ def test_pipelined_tiles_transposes(tile, ta, tb):
     check_mm(oC, dC, oA, dA, ta, oB, dB, tb, 1e-6, 0.75, 0.0, tile=tile)
 
 
+@pytest.mark.parametrize("ta", [0, 1])
+def test_many_small_blocks(ta):
+    """A 100 x 100 grid of 3-blocks with 70 inner blocks: more than one scan
+    tile for the screening's segment, arena and output scans (10,000 C
+    blocks, 10,000 padded output slots), and more than one 64-wide step in
+    the per-row lists of op(A)'s blocks."""
+    rng = np.random.default_rng(77 + ta)
+    ni, nk, nj, bs = 300, 210, 300, 3
+    a_shape = (nk, ni) if ta else (ni, nk)
+    a = rand_dense(rng, synth.uniform_sizes(a_shape[0], bs), synth.uniform_sizes(a_shape[1], bs), 0.5, 0.3)
+    b = rand_dense(rng, synth.uniform_sizes(nk, bs), synth.uniform_sizes(nj, bs), 0.5, 0.3)
+    oA, dA = pair_uniform(a, bs, bs, 1e-12)
+    oB, dB = pair_uniform(b, bs, bs, 1e-12)
+    oC, dC = empty_pair_uniform(ni, nj, bs, bs)
+    n, _ = check_mm(oC, dC, oA, dA, ta, oB, dB, 0, 1e-3, 1.0, 0.0)
+    assert n > 0
+
+
 def test_binding_threshold_screens_tasks():
     cfg = synth.band_workload(n=1024, bs=64, band=6, decay=2.5)
     a = synth.dense_from_scales(1, cfg.rsz, cfg.ksz, cfg.a_scale)
