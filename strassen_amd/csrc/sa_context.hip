@@ -3,6 +3,8 @@
 #include <cstdarg>
 #include <cstring>
 
+#include <cstdlib>
+
 #include "sa_internal.hpp"
 
 namespace sa {
@@ -220,6 +222,56 @@ int64_t scan_exclusive_i64(sa_context* c, int64_t* d, int64_t n, bool read_total
   SA_CHECK_HIP(hipGetLastError());
   if (!read_total) return 0;
   return read_i64(c, total_slot);
+}
+
+// SA_GRAPH_DEBUG=1: say on stderr why a graph fell back to eager launches
+static void graph_note(const char* what, hipError_t e) {
+  static const bool on = std::getenv("SA_GRAPH_DEBUG") != nullptr;
+  if (on) std::fprintf(stderr, "strassen_amd: %s: %s\n", what, hipGetErrorString(e));
+}
+
+void run_as_graph(sa_context* c, const std::function<void()>& f) {
+  const hipError_t be = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+  if (be != hipSuccess) {
+    graph_note("stream capture unavailable", be);
+    (void)hipGetLastError();
+    f();  // capture unavailable: the same launches, eagerly
+    return;
+  }
+  hipGraph_t g = nullptr;
+  try {
+    f();
+  } catch (...) {
+    (void)hipStreamEndCapture(c->stream, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  const hipError_t ee = hipStreamEndCapture(c->stream, &g);
+  if (ee != hipSuccess || !g) {
+    graph_note("stream capture failed", ee);
+    (void)hipGetLastError();
+    f();
+    return;
+  }
+  hipGraphExec_t e = nullptr;
+  const hipError_t ie = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+  if (ie != hipSuccess) {
+    graph_note("graph instantiation failed", ie);
+    (void)hipGetLastError();
+    (void)hipGraphDestroy(g);
+    f();  // the captured launches did not run: run them eagerly
+    return;
+  }
+  hipError_t le;
+  {
+    ProfScope ps(c, "graph");
+    le = hipGraphLaunch(e, c->stream);
+  }
+  const hipError_t se = hipStreamSynchronize(c->stream);
+  (void)hipGraphExecDestroy(e);
+  (void)hipGraphDestroy(g);
+  SA_CHECK_HIP(le);
+  SA_CHECK_HIP(se);
 }
 
 int64_t read_i64(sa_context* c, const int64_t* d) {

@@ -415,7 +415,9 @@ static int heuristic_cfg(const GemmBatch& b) {
   return 4;
 }
 
-static void launch_cfg(sa_context* c, const GemmBatch& b, int cfg) {
+// work items of one tile config (output tiles when a block exceeds the tile;
+// sizing reads back one total)
+static GemmPlan plan_items(sa_context* c, const GemmBatch& b, int cfg) {
   const Cfg& k = kCfgs[cfg];
   const GemmItem* items = nullptr;
   int64_t nitems = b.nouts;
@@ -431,11 +433,20 @@ static void launch_cfg(sa_context* c, const GemmBatch& b, int cfg) {
     items = c->ws_gemm_items2.as<GemmItem>();
   }
   SA_REQUIRE(nitems < (int64_t)1 << 31, "too many GEMM work items");
-  const int64_t nwg = ceil_div(nitems, k.items_per_wg);
+  return GemmPlan{cfg, items, nitems};
+}
+
+// one kernel launch, no host interaction (capturable)
+void gemm_fp64_launch(sa_context* c, const GemmBatch& b, const GemmPlan& g) {
+  if (b.nouts == 0) return;
+  const Cfg& k = kCfgs[g.cfg];
+  const int64_t nwg = ceil_div(g.nitems, k.items_per_wg);
   hipLaunchKernelGGL(k.fn[b.ta * 2 + b.tb], dim3((unsigned)nwg), dim3(k.threads), 0, c->stream, b.outs, b.prods,
-                     items, nitems, b.alpha);
+                     g.items, g.nitems, b.alpha);
   SA_CHECK_HIP(hipGetLastError());
 }
+
+static void launch_cfg(sa_context* c, const GemmBatch& b, int cfg) { gemm_fp64_launch(c, b, plan_items(c, b, cfg)); }
 
 // ---------------------------------------------------------------- autotuner --
 // The tile configuration is chosen per shape class (transposes, largest block
@@ -523,12 +534,18 @@ static int autotune(sa_context* c, const GemmBatch& b) {
   return best;
 }
 
-void gemm_fp64_run(sa_context* c, const GemmBatch& b, int cfg) {
-  if (b.nouts == 0) return;
+GemmPlan gemm_fp64_plan(sa_context* c, const GemmBatch& b, int cfg) {
+  if (b.nouts == 0) return GemmPlan{};
   if (cfg < 0 || cfg >= kNumCfgs) cfg = autotune(c, b);
   c->last_gemm_cfg = cfg;
+  return plan_items(c, b, cfg);
+}
+
+void gemm_fp64_run(sa_context* c, const GemmBatch& b, int cfg) {
+  if (b.nouts == 0) return;
+  const GemmPlan g = gemm_fp64_plan(c, b, cfg);
   ProfScope ps(c, "gemm_fp64");
-  launch_cfg(c, b, cfg);
+  gemm_fp64_launch(c, b, g);
 }
 
 }  // namespace sa

@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <map>
 #include <string>
+#include <functional>
 #include <vector>
 
 #include "../../include/strassen_amd.h"
@@ -318,5 +319,18 @@ int gemm_fp64_num_configs();
 const char* gemm_fp64_config_name(int cfg);
 // Run a batch with tile config cfg (-1: autotuned choice for this shape class).
 void gemm_fp64_run(sa_context* c, const GemmBatch& b, int cfg);
+// the same in two halves: the tile config (autotuned when cfg < 0) and the
+// work-item list, which may synchronise; then one launch with no host
+// interaction, which a stream capture can record
+struct GemmPlan {
+  int cfg = 0;
+  const GemmItem* items = nullptr;
+  int64_t nitems = 0;
+};
+GemmPlan gemm_fp64_plan(sa_context* c, const GemmBatch& b, int cfg);
+void gemm_fp64_launch(sa_context* c, const GemmBatch& b, const GemmPlan& g);
+// run f's launches on c's stream as one HIP graph (captured, instantiated,
+// launched once); f must not synchronise or copy from the host
+void run_as_graph(sa_context* c, const std::function<void()>& f);
 
 }  // namespace sa

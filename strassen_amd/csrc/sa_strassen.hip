@@ -567,12 +567,8 @@ static void run_plan(sa_bsm* C, sa_bsm* A, sa_bsm* B, Plan& p, int tile_cfg) {
   DevBuf& bprods = c->buf("st_prods");
   bouts.reserve(sizeof(GemmOut) * (size_t)(p.nouts + 1));
   bprods.reserve(sizeof(GemmProd) * (size_t)(p.nprods + 1));
-  {
-    ProfScope ps(c, "winograd");
-    for (size_t d = 0; d < p.depth.size(); ++d)
-      hipLaunchKernelGGL(k_preadd, dim3((unsigned)pre[d].size()), dim3(256), 0, c->stream, d_nodes,
-                         (const PosItem*)d_pre[d], P);
-  }
+  // leaf records (they only point into A, B, C and the scratch), then the
+  // leaf GEMM's tile config and work items, which may synchronise
   if (!p.leaves.empty())
     hipLaunchKernelGGL(k_leaf_records, dim3((unsigned)p.leaves.size()), dim3(256), 0, c->stream, d_leaves, P,
                        (int32_t)bs, (const double*)dum.p, bouts.as<GemmOut>(), bprods.as<GemmProd>());
@@ -583,7 +579,7 @@ static void run_plan(sa_bsm* C, sa_bsm* A, sa_bsm* B, Plan& p, int tile_cfg) {
   gb.nprods = p.nprods;
   gb.alpha = 1.0;
   gb.max_m = gb.max_n = gb.max_k = (int32_t)bs;
-  gemm_fp64_run(c, gb, tile_cfg);
+  const GemmPlan gp = gemm_fp64_plan(c, gb, tile_cfg);
   // contributions of the roots (depth 0) grouped by target C block, in root order
   std::vector<std::pair<int64_t, RootContrib>> rc;
   for (int64_t n = depth_first[0]; n < depth_first[1]; ++n) {
@@ -610,16 +606,24 @@ static void run_plan(sa_bsm* C, sa_bsm* A, sa_bsm* B, Plan& p, int tile_cfg) {
   const int64_t* d_cblk = upload(c, "st_cblk", cblk);
   const int64_t* d_start = upload(c, "st_cstart", start);
   const RootContrib* d_contrib = upload(c, "st_contrib", contrib);
+  // every level of the recursion as one HIP graph: the pre-adds per depth, the
+  // grouped GEMM of all leaves, the post-adds per depth and the roots' sums
   {
     ProfScope ps(c, "winograd");
-    for (size_t d = p.depth.size(); d-- > 1;)
-      hipLaunchKernelGGL(k_postadd, dim3((unsigned)post[d].size()), dim3(256), 0, c->stream, d_nodes,
-                         (const PosItem*)d_post[d], P);
-    if (!cblk.empty())
-      hipLaunchKernelGGL(k_postadd_roots, dim3((unsigned)cblk.size()), dim3(256), 0, c->stream, d_nodes, d_cblk,
-                         d_start, d_contrib, P);
+    run_as_graph(c, [&] {
+      for (size_t d = 0; d < p.depth.size(); ++d)
+        hipLaunchKernelGGL(k_preadd, dim3((unsigned)pre[d].size()), dim3(256), 0, c->stream, d_nodes,
+                           (const PosItem*)d_pre[d], P);
+      gemm_fp64_launch(c, gb, gp);
+      for (size_t d = p.depth.size(); d-- > 1;)
+        hipLaunchKernelGGL(k_postadd, dim3((unsigned)post[d].size()), dim3(256), 0, c->stream, d_nodes,
+                           (const PosItem*)d_post[d], P);
+      if (!cblk.empty())
+        hipLaunchKernelGGL(k_postadd_roots, dim3((unsigned)cblk.size()), dim3(256), 0, c->stream, d_nodes, d_cblk,
+                           d_start, d_contrib, P);
+      SA_CHECK_HIP(hipGetLastError());
+    });
   }
-  SA_CHECK_HIP(hipGetLastError());
   SA_CHECK_HIP(hipStreamSynchronize(c->stream));  // host item vectors go out of scope
 }
 

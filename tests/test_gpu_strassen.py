@@ -92,6 +92,31 @@ def test_random_inputs_vs_oracle(nb, bs, cutoff, kind, levels):
     assert rel_fro(cd, a @ b + 0.25) < 1e-12
 
 
+def test_levels_run_as_one_graph_repeatedly():
+    """The execution phase (pre-adds, leaf GEMM, post-adds) runs as one HIP
+    graph per call: three calls in a row on new integer inputs (and eager
+    screened multiplies between them) stay exact, one graph launch each."""
+    c = ctx()
+    c.set_profiling(True)
+    c.profile_reset()
+    try:
+        for seed in range(3):
+            rng = np.random.default_rng(90 + seed)
+            n, a, b = inputs(rng, 16, 4, "dense", integer=True)
+            co, cd, st = run_both(a, b, n, 4, 0.0, 0, oracle.forced_tree(16, {1, 2}))
+            assert st["nroots"] > 0
+            assert np.array_equal(cd, a @ b)
+            oA, dA = pair_uniform(a, 4, 4, 0.0)
+            oB, dB = pair_uniform(b, 4, 4, 0.0)
+            dC = sa.BSM.uniform(c, n, n, 4, 4, 0.0)
+            sa.matmult(dC, dA, 0, dB, 0, 0.0, 1.0, 0.0)
+            assert np.array_equal(dC.to_dense(), a @ b)
+        c.synchronize()
+        assert c.profile("graph")[1] == 3
+    finally:
+        c.set_profiling(False)
+
+
 def random_plan(rng, nb, p_s, p_d):
     sizes = oracle.tree_level_sizes(nb)
     return ([(rng.random(s) < p_s).astype(np.uint8) for s in sizes],
