@@ -139,6 +139,14 @@ int sa_bsm_fill(sa_bsm_t m, double v);
 int sa_bsm_zero(sa_bsm_t m);
 /* scale / *= (blocksparsematrix.hpp:71-77, 250-254): cached norm *= s*s. */
 int sa_bsm_scale(sa_bsm_t m, double s);
+/* scale with a choice of cached-norm update: SA_NORM_RULE_REFERENCE is
+ * sa_bsm_scale (the reference's norm *= s*s, blocksparsematrix.hpp:75-76);
+ * SA_NORM_RULE_RECOMPUTE recomputes each present block's norm from its scaled
+ * values in the same pass (bit-identical to sa_bsm_calc_norms afterwards), the
+ * corrected rule |s| * norm up to rounding.  No counterpart in the reference. */
+#define SA_NORM_RULE_REFERENCE 0
+#define SA_NORM_RULE_RECOMPUTE 1
+int sa_bsm_scale_ex(sa_bsm_t m, double s, int32_t norm_rule);
 /* calc_frobenius_norms (blocksparsematrix.hpp:139-147). */
 int sa_bsm_calc_norms(sa_bsm_t m);
 /* recompress (blocksparsematrix.hpp:149-156). */

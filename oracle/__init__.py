@@ -102,6 +102,10 @@ def ref():
         R.ref_strassen.argtypes = [P, P, P, I64, I32, D, D, I64, P]
         R.ref_occ_ri_k.restype = C.c_int
         R.ref_occ_ri_k.argtypes = [I64, I64, I64, I64, P, P, P, I32, D, P]
+        R.ref_scale_recompress.restype = C.c_int
+        R.ref_scale_recompress.argtypes = [P, I64, I64, I32, I32, D, D, P, P, P]
+        R.ref_dot.restype = D
+        R.ref_dot.argtypes = [P, P, I64, I64, I32, I32, D, D]
         _ref = R
     return _ref
 

@@ -76,6 +76,27 @@ int ref_matmult(const double* a, int64_t ar, int64_t ac, double a_thresh,
   return 0;
 }
 
+// Uniform BSM built from `a` at threshold thresh (copy-in drop rule), then
+// scale(s) (the cached norms follow the reference's rule), then recompress();
+// returns presence, cached norms and the dense result.
+int ref_scale_recompress(const double* a, int64_t nr, int64_t nc, int32_t bs_r, int32_t bs_c, double thresh,
+                         double s, uint8_t* present, double* norms, double* dense) {
+  BSM m(a, nr, nc, bs_r, bs_c, thresh);
+  m.scale(s);
+  m.recompress();
+  dump(m, dense, present, norms);
+  return 0;
+}
+
+// dot(A, B, thresh) of two uniform BSMs built from a and b (copy-in at
+// build_thresh).
+double ref_dot(const double* a, const double* b, int64_t nr, int64_t nc, int32_t bs_r, int32_t bs_c,
+               double build_thresh, double thresh) {
+  BSM A(a, nr, nc, bs_r, bs_c, build_thresh);
+  BSM B(b, nr, nc, bs_r, bs_c, build_thresh);
+  return dot(A, B, thresh);
+}
+
 // matmult_strassen_dryrun: concatenated level bits; returns max_level.
 int ref_dryrun(const double* a, const double* b, int64_t n, int32_t bs, double bsm_thresh,
                double thresh_per_block, uint8_t* bits, int64_t cap) {

@@ -20,6 +20,7 @@ LIB_PATH = os.path.join(PKG, "lib", "libstrassen_amd.so")
 
 SA_OK = 0
 MODE_FP64, MODE_OZAKI = 0, 1
+OK, ERR_INVALID, ERR_HIP, ERR_OOM, ERR_UNSUPPORTED = 0, -1, -2, -3, -4  # SA_OK, SA_ERR_*
 SLICE_FP16, SLICE_BF16, SLICE_INT8 = 0, 1, 2
 SCHED_HEURISTIC, SCHED_MLP, SCHED_TREE = 0, 1, 2
 
@@ -88,6 +89,7 @@ ABI = {
     "sa_bsm_fill": (INT, [P, D]),
     "sa_bsm_zero": (INT, [P]),
     "sa_bsm_scale": (INT, [P, D]),
+    "sa_bsm_scale_ex": (INT, [P, D, INT]),
     "sa_bsm_calc_norms": (INT, [P]),
     "sa_bsm_recompress": (INT, [P]),
     "sa_bsm_add": (INT, [P, P, D]),
@@ -315,8 +317,13 @@ class BSM:
     def zero(self):
         _check(lib().sa_bsm_zero(self.h))
 
-    def scale(self, s):
-        _check(lib().sa_bsm_scale(self.h, s))
+    def scale(self, s, recompute_norms=False):
+        """scale (blocksparsematrix.hpp:71-77: cached norms *= s*s), or with
+        recompute_norms the cached norms recomputed from the scaled blocks."""
+        if recompute_norms:
+            _check(lib().sa_bsm_scale_ex(self.h, s, 1))
+        else:
+            _check(lib().sa_bsm_scale(self.h, s))
 
     def calc_norms(self):
         _check(lib().sa_bsm_calc_norms(self.h))

@@ -72,15 +72,17 @@ def build_cpp() -> list:
     os.makedirs(BIN, exist_ok=True)
     out = []
 
-    def one(name):
-        exe = os.path.join(BIN, name)
+    def one(src):
+        exe = os.path.join(BIN, os.path.splitext(os.path.basename(src))[0])
         _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), "-I", CPP,
-              os.path.join(CPP, "drivers", name + ".cpp"), "-L", LIBDIR, "-lstrassen_amd",
-              "-Wl,-rpath,$ORIGIN/../lib", "-o", exe])
+              src, "-L", LIBDIR, "-lstrassen_amd", "-Wl,-rpath,$ORIGIN/../lib", "-o", exe])
         return exe
 
-    with cf.ThreadPoolExecutor(max_workers=3) as ex:
-        out = list(ex.map(one, DRIVERS))
+    # the reference-argv drivers, and the C++ programs the GPU tests drive
+    srcs = [os.path.join(CPP, "drivers", name + ".cpp") for name in DRIVERS]
+    srcs += sorted(glob.glob(os.path.join(ROOT, "tests", "cpp", "*.cpp")))
+    with cf.ThreadPoolExecutor(max_workers=4) as ex:
+        out = list(ex.map(one, srcs))
     return out
 
 

@@ -3,8 +3,10 @@
 // The reference only declares them; the semantics implemented are:
 //   form 1:  C_ij = alpha * sum_{k,l} op(A^l)_ik op(B^l)_kj + beta C_ij
 //   form 2:  op_C(C^l)_ij = alpha * sum_k op(A)_ik op(B^l)_kj + beta op_C(C^l)_ij
-// Each term is screened like matmult; form 1 runs the L products as one
-// batched accumulation on the device context.
+// Each term is one screened device matmult (the slices are separate matrices
+// with their own block arenas, so form 1 accumulates slice by slice into C in
+// l order; the RI-K step's K contraction, sa_ri_k_step, is the form-1 sum run
+// as one screened multiply over a reshape view of a single arena).
 #ifndef SA_BLOCKSPARSE3D_TENSOR_H
 #define SA_BLOCKSPARSE3D_TENSOR_H
 #include <vector>
@@ -41,6 +43,16 @@ void tensormult(BlockSparse3DTensor<Num>& C, const bool transC, const BlockSpars
       matmult(C[l], B[l], !transB, A, !transA, thresh, alpha, beta);
     }
   }
+}
+
+// the reference's fourth overload (blocksparse3d_tensor.h:35-43): it forwards
+// to form 2 with transC = true and both operand flags flipped, i.e.
+// C^l = alpha op(B^l) op(A) + beta C^l; kept with that forwarding
+template <typename Num>
+void tensormult(BlockSparse3DTensor<Num>& C, const BlockSparseMatrix<Num>& A, const bool transA,
+                const BlockSparse3DTensor<Num>& B, const bool transB, const Num thresh, const Num& alpha,
+                const Num& beta) {
+  tensormult(C, true, A, !transA, B, !transB, thresh, alpha, beta);
 }
 
 #endif

@@ -86,6 +86,12 @@ class BlockSparseMatrix final {
   }
   void fill_with_values(const Num val) { sa_cpp::check(sa_bsm_fill(_h, val), "sa_bsm_fill"); }
   void scale(Num s) { sa_cpp::check(sa_bsm_scale(_h, s), "sa_bsm_scale"); }
+  // extension: recompute_norms = true refreshes the cached norms from the
+  // scaled blocks instead of the reference's norm *= s*s
+  void scale(Num s, bool recompute_norms) {
+    sa_cpp::check(sa_bsm_scale_ex(_h, s, recompute_norms ? SA_NORM_RULE_RECOMPUTE : SA_NORM_RULE_REFERENCE),
+                  "sa_bsm_scale_ex");
+  }
   void zero() { sa_cpp::check(sa_bsm_zero(_h), "sa_bsm_zero"); }
   // blocksparsematrix.hpp:103-124: only present blocks are written
   void to_pointer(Num* out) const { sa_cpp::check(sa_bsm_to_dense(_h, out, (int64_t)_nrow), "sa_bsm_to_dense"); }

@@ -222,6 +222,31 @@ __global__ void __launch_bounds__(256) k_scale_blocks(Geo g, const int64_t* __re
   for (int64_t e = threadIdx.x; e < n; e += blockDim.x) d[e] = d[e] * s;
 }
 
+// Scale every present block and recompute its cached norm from the scaled
+// values in the canonical order (one wave per block, lane l over e = l, l+64,
+// ...): the result equals calc_frobenius_norms() after the scale.
+__global__ void __launch_bounds__(64) k_scale_blocks_renorm(Geo g, const int64_t* __restrict__ off, double s,
+                                                            double* __restrict__ arena, double* __restrict__ norm) {
+  const int64_t b = blockIdx.x;
+  const int64_t o = off[b];
+  if (o < 0) {
+    if (threadIdx.x == 0) norm[b] = 0.0;
+    return;
+  }
+  const int rb = (int)(b % g.nrb), cb = (int)(b / g.nrb);
+  const int64_t n = (int64_t)g.rsz[rb] * g.csz[cb];
+  double* d = arena + o;
+  double a = 0.0;
+  for (int64_t e = threadIdx.x; e < n; e += 64) {
+    const double y = d[e] * s;
+    d[e] = y;
+    a = fma(y, y, a);
+  }
+#pragma unroll
+  for (int off2 = 32; off2 >= 1; off2 >>= 1) a = a + __shfl_down(a, off2, 64);
+  if (threadIdx.x == 0) norm[b] = sqrt(a);
+}
+
 __global__ void k_recompress(Geo g, int64_t nb, double thresh, int64_t* __restrict__ off,
                              double* __restrict__ norm) {
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -710,6 +735,24 @@ int sa_bsm_scale(sa_bsm_t m, double s) {
     if (m->nb() == 0) return;
     hipLaunchKernelGGL(k_scale_blocks, dim3((unsigned)m->nb()), dim3(256), 0, m->ctx->stream, geo(m),
                        (const int64_t*)m->d_off, s, m->data(), m->d_norm);
+  });
+}
+
+int sa_bsm_scale_ex(sa_bsm_t m, double s, int32_t norm_rule) {
+  return guard([&] {
+    SA_REQUIRE(m, "null matrix");
+    SA_REQUIRE(norm_rule == SA_NORM_RULE_REFERENCE || norm_rule == SA_NORM_RULE_RECOMPUTE,
+               "scale: unknown norm rule %d", norm_rule);
+    if (norm_rule == SA_NORM_RULE_REFERENCE) {
+      const int rc = sa_bsm_scale(m, s);
+      if (rc != SA_OK) throw Error{rc};
+      return;
+    }
+    SA_REQUIRE_WRITABLE(m);
+    if (m->nb() == 0) return;
+    hipLaunchKernelGGL(k_scale_blocks_renorm, dim3((unsigned)m->nb()), dim3(64), 0, m->ctx->stream, geo(m),
+                       (const int64_t*)m->d_off, s, m->data(), m->d_norm);
+    SA_CHECK_HIP(hipGetLastError());
   });
 }
 

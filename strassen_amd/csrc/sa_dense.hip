@@ -100,10 +100,14 @@ int sa_dgemm(sa_ctx_t c, int transA, int transB, int64_t m, int64_t n, int64_t k
       SA_CHECK_HIP(hipMemcpyAsync(bA.p, A, sa_, hipMemcpyHostToDevice, c->stream));
       SA_CHECK_HIP(hipMemcpyAsync(bB.p, B, sb_, hipMemcpyHostToDevice, c->stream));
     }
-    if (beta != 0.0) SA_CHECK_HIP(hipMemcpyAsync(bC.p, Cm, sc_, hipMemcpyHostToDevice, c->stream));
+    // C moves as its m x n part only: rows m..ldc-1 of each column belong to
+    // the caller and are neither read nor written
+    const size_t pitch = sizeof(double) * (size_t)ldc, width = sizeof(double) * (size_t)m;
+    if (beta != 0.0)
+      SA_CHECK_HIP(hipMemcpy2DAsync(bC.p, pitch, Cm, pitch, width, (size_t)n, hipMemcpyHostToDevice, c->stream));
     device_dgemm(c, transA != 0, transB != 0, m, n, k, alpha, bA.as<double>(), lda, bB.as<double>(), ldb, beta,
                  bC.as<double>(), ldc);
-    SA_CHECK_HIP(hipMemcpyAsync(Cm, bC.p, sc_, hipMemcpyDeviceToHost, c->stream));
+    SA_CHECK_HIP(hipMemcpy2DAsync(Cm, pitch, bC.p, pitch, width, (size_t)n, hipMemcpyDeviceToHost, c->stream));
     SA_CHECK_HIP(hipStreamSynchronize(c->stream));
   });
 }

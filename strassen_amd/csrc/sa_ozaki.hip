@@ -21,9 +21,12 @@
 //    folded every 2 (FP16) or 8 (BF16) k-steps.  Later weights may round in
 //    FP32, which costs at most 2^-(2 beta + 21) of the leading term per fold.
 // INT8 slices (an MI355X-first third form, v_mfma_i32_16x16x64_i8 at twice
-// the FP16 rate): beta = 8, TS = 2 digits in int8; digit products accumulate
-// in int32, exact for 2^11 / S k-steps of 64 (|d_p d_q| <= 2^14), so products
-// are folded only at their end (or every 256 k-steps).
+// the FP16 rate): beta = 8, TS = 2 digits in int8, where a vector whose
+// largest value leaves the leading digit of m = n + H at most 127 takes the
+// exponent one lower (TS = 1 in effect: one more significant bit); digit
+// products accumulate in int32, exact for 2^11 / S k-steps of 64
+// (|d_p d_q| <= 2^14), so products are folded only at their end (or every
+// 256 k-steps).
 // Truncation error per product is about 2^-(beta*S - TS) relative to the
 // row/column maxima of the two blocks (4 FP16 slices: 2^-38; 5 INT8: 2^-38).
 //
@@ -200,6 +203,9 @@ __global__ void __launch_bounds__(64) k_slice(SliceArgs s, int S, const int64_t*
     }
   };
 
+  const int64_t mask = ((int64_t)1 << BETA) - 1, half = (int64_t)1 << (BETA - 1);
+  int64_t H = 0;
+  for (int j = 0; j < S - 1; ++j) H += half << (BETA * j);
   int e = 0;
   if (mode == 2) {
     e = exps[ex_off[b] + v0 + vl];  // >= this block's own exponent: digits stay in range
@@ -214,15 +220,20 @@ __global__ void __launch_bounds__(64) k_slice(SliceArgs s, int S, const int64_t*
     }
     mx = fmax(mx, __shfl_xor(mx, 16, 64));
     mx = fmax(mx, __shfl_xor(mx, 32, 64));
-    if (mx > 0.0) (void)frexp(mx, &e);  // mx < 2^e
-    else if (mode == 1) e = kNoExp;     // a zero vector does not raise its line's exponent
+    if (mx > 0.0) {
+      (void)frexp(mx, &e);  // mx < 2^e
+      // INT8: one guard bit of the leading digit is needed only when the
+      // vector's largest value would carry it to +128; otherwise the scale
+      // moves up one bit (every |x| <= mx then gives a leading digit in
+      // [-128, 127]), which halves the truncation of the vector's digits
+      if (KIND == 2 && (int64_t)rint(ldexp(mx, BETA * S - 1 - e)) + H < ((int64_t)1 << (BETA * S - 1))) e -= 1;
+    } else if (mode == 1) {
+      e = kNoExp;  // a zero vector does not raise its line's exponent
+    }
     if (kg == 0) exps[ex_off[b] + v0 + vl] = e;
     if (mode == 1) return;
   }
 
-  const int64_t mask = ((int64_t)1 << BETA) - 1, half = (int64_t)1 << (BETA - 1);
-  int64_t H = 0;
-  for (int j = 0; j < S - 1; ++j) H += half << (BETA * j);
   for (int ks = 0; ks < nks; ++ks) {
     stage(ks);
     __syncthreads();

@@ -24,6 +24,7 @@
 //  that fit a scratch budget.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "sa_internal.hpp"
@@ -671,6 +672,20 @@ __global__ void k_scatter_regions(const int64_t* __restrict__ coff, double* __re
   }
 }
 
+// Scratch budget of one batch: free HBM / frac, capped at SA_SCRATCH_MB
+// megabytes when that environment variable is set (smaller batches for
+// callers that share the device, and for tests of the batching itself).
+static int64_t scratch_budget(int frac) {
+  size_t free_b = 0, total_b = 0;
+  SA_CHECK_HIP(hipMemGetInfo(&free_b, &total_b));
+  int64_t b = (int64_t)(free_b / (size_t)frac);
+  if (const char* v = std::getenv("SA_SCRATCH_MB")) {
+    const int64_t cap = std::atoll(v) << 20;
+    if (cap > 0) b = std::min(b, cap);
+  }
+  return b;
+}
+
 // dense roots of one level; returns the number executed
 static int64_t run_dense_level(sa_bsm* C, sa_bsm* A, sa_bsm* B, std::vector<Root> rs) {
   if (rs.empty()) return 0;
@@ -683,10 +698,8 @@ static int64_t run_dense_level(sa_bsm* C, sa_bsm* A, sa_bsm* B, std::vector<Root
     if (x.ib0 != y.ib0) return x.ib0 < y.ib0;
     return x.kb0 < y.kb0;
   });
-  size_t free_b = 0, total_b = 0;
-  SA_CHECK_HIP(hipMemGetInfo(&free_b, &total_b));
   const int64_t per = 3 * W2 * (int64_t)sizeof(double);
-  const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>((int64_t)(free_b / 4) / per, 4096));
+  const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(scratch_budget(4) / per, 4096));
   DevBuf& bp = c->buf("dn_pack");
   bp.reserve((size_t)(std::min<int64_t>(chunk, (int64_t)rs.size()) * per) + 64);
   ProfScope ps(c, "dense_roots");
@@ -1078,9 +1091,7 @@ static void hybrid(sa_bsm* C, sa_bsm* A, sa_bsm* B, double thresh, const sa_opts
     return;
   }
   // batches of roots whose scratch fits the budget
-  size_t free_b = 0, total_b = 0;
-  SA_CHECK_HIP(hipMemGetInfo(&free_b, &total_b));
-  const int64_t budget = (int64_t)(free_b / 2);
+  const int64_t budget = scratch_budget(2);
   const int64_t bs2b = bs * bs * (int64_t)sizeof(double);
   size_t i = 0;
   while (i < roots.size()) {

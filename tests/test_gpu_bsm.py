@@ -114,3 +114,51 @@ def test_error_paths_on_device():
         sa.matmult(d, d, 0, d, 0, 0.0)  # aliasing C and A
     with pytest.raises(sa.SAError):
         sa.BSM(ctx(), [3, 0], [2], 0.0)  # zero-sized block row
+
+
+def test_scale_recompress_designed_ties_on_device():
+    """Device scale + recompress on blocks exactly at the target-size threshold
+    and ragged blocks between the target- and actual-size thresholds: the kept
+    set and cached norms are fixed by construction (tests/designed.py)."""
+    import designed
+
+    a, thr, s, pres, norms = designed.recompress_case()
+    d = sa.BSM.uniform(ctx(), designed.NR, designed.NC, designed.BS, designed.BS, thr).from_dense(a)
+    d.scale(s)
+    d.recompress()
+    p, n = d.meta()
+    assert np.array_equal(p, pres)
+    assert np.array_equal(n[p], norms[p])
+
+
+def test_dot_thresholded_designed_ties_on_device():
+    import designed
+
+    a, b, thr, expect, expect_all = designed.dot_case()
+    A = sa.BSM.uniform(ctx(), designed.NR, designed.NC, designed.BS, designed.BS, 0.0).from_dense(a)
+    B = sa.BSM.uniform(ctx(), designed.NR, designed.NC, designed.BS, designed.BS, 0.0).from_dense(b)
+    assert A.dot(B, thr) == expect
+    assert A.dot(B, 0.0) == expect_all
+
+
+def test_scale_with_recomputed_norms():
+    """The opt-in corrected norm rule: after scale(s, recompute_norms) the cached
+    norms are the canonical norms of the scaled blocks (equal to a later
+    calc_norms, and to |s| times the old norm up to rounding); the default
+    keeps the reference's s*s."""
+    rng = np.random.default_rng(23)
+    a = rand_dense(rng, [7] * 5 + [3], [9] * 4 + [2], 0.6)
+    for s in [-2.5, 0.3, 1e-3]:
+        o, d = pair_uniform(a, 7, 9, 0.0)
+        _, n0 = d.meta()
+        d.scale(s, recompute_norms=True)
+        p1, n1 = d.meta()
+        d.calc_norms()
+        _, n2 = d.meta()
+        assert np.array_equal(n1, n2)
+        np.testing.assert_allclose(n1[p1], abs(s) * n0[p1], rtol=1e-14)
+        o.scale(s)
+        assert np.array_equal(d.to_dense(), o.to_dense())
+        d2 = sa.BSM.uniform(ctx(), a.shape[0], a.shape[1], 7, 9, 0.0).from_dense(a)
+        d2.scale(s)
+        assert_same_meta(o, d2)  # the reference rule, bit-exact with the oracle

@@ -185,3 +185,44 @@ def test_kwindows_match_single_launch(monkeypatch, beta):
     check_mm(oD, dD, oA, dA, 0, oB, dB, 0, cfg.thresh, 1.0, 0.0)
     assert ctx().kwindows() == 1
 
+
+
+def test_near_tie_task_list_bit_exact():
+    """Screening at near-ties (tests/designed.py: all block norms equal in exact
+    arithmetic, threshold at the median estimate): the device's task list is
+    the oracle's, triple for triple, because both screen with the same
+    canonical block norms and the same FP64 product and comparison."""
+    import designed
+
+    a, b, bs = designed.near_tie_case()
+    n = a.shape[0]
+    oA, dA = pair_uniform(a, bs, bs, 0.0)
+    oB, dB = pair_uniform(b, bs, bs, 0.0)
+    _, na = oA.meta(); _, nbm = oB.meta()
+    est = (na[:, :, None] * nbm[None, :, :]).ravel()
+    thresh = float(np.median(np.sort(est)[: est.size // 2 * 2 - 1])) / float(bs ** 3)
+    oC = oracle.OBsm.uniform(n, n, bs, bs, 0.0)
+    t_or = oracle.matmult(oC, oA, 0, oB, 0, thresh)
+    dC = sa.BSM.uniform(ctx(), n, n, bs, bs, 0.0)
+    sa.matmult(dC, dA, 0, dB, 0, thresh)
+    t_dev = ctx().tasklist()
+    assert 0 < len(t_or) < (n // bs) ** 3
+    assert np.array_equal(t_or, t_dev)
+    assert rel_fro(dC.to_dense(), oC.to_dense()) < 1e-13
+
+
+def test_uneven_atoms_full_size():
+    """The uneven-atoms workload at full size (n = 12000, 5-40 atom blocks,
+    10 % fill, ~2.4e6 screened products): operands generated in HBM and read
+    back for the oracle (the generator is pinned to its host definition in
+    test_gpu_bsm), task list bit-exact, C within 1e-12."""
+    w = synth.atoms_workload()
+    dA = sa.BSM(ctx(), w.rsz, w.ksz, w.thresh).generate(1, w.a_scale)
+    dB = sa.BSM(ctx(), w.ksz, w.csz, w.thresh).generate(2, w.b_scale)
+    oA = oracle.OBsm(w.rsz, w.ksz, w.thresh).from_dense(dA.to_dense())
+    oB = oracle.OBsm(w.ksz, w.csz, w.thresh).from_dense(dB.to_dense())
+    assert_same_meta(oA, dA)
+    assert_same_meta(oB, dB)
+    oC, dC = empty_pair_blocked(w.rsz, w.csz)
+    n, err = check_mm(oC, dC, oA, dA, 0, oB, dB, 0, w.thresh, 1.0, 0.0)
+    assert n > 2_000_000

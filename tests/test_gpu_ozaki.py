@@ -87,7 +87,7 @@ def test_deep_k_fold_schedule(stype, slices):
 @pytest.mark.parametrize("slices", [4, 5, 6])
 def test_int8_extreme_digits_short_products(slices):
     """Entries whose 5-slice INT8 digits sit at the ends of their ranges
-    (leading digit +-63/64, the others -128 or 127) in products of 1 to 4
+    (leading digit near +-127, the others -128 or 127) in products of 1 to 4
     k-steps, against a host restatement of the split product (exact integer
     slice sums, FP64 only in the fold)."""
     rng = np.random.default_rng(50 + slices)
@@ -111,16 +111,22 @@ def test_int8_extreme_digits_short_products(slices):
 
 def int8_slice_model(a, rsz, ksz, b, csz, S):
     """Split product with INT8 digits restated on the host: per row of an
-    A block / column of a B block the scale 2^e above its largest entry,
-    n = rint(x 2^(8S-2-e)), digits of n + H (leading one m >> 8(S-1), the
-    others byte - 128), slice pairs p + q <= S - 1 summed exactly in int64,
-    folded into FP64 per product in k order."""
+    A block / column of a B block the scale 2^e above its largest entry mx,
+    lowered to 2^(e-1) when rint(mx 2^(8S-1-e)) + H < 2^(8S-1) (the leading
+    digit then still fits int8), n = rint(x 2^(8S-2-e)), digits of n + H
+    (leading one m >> 8(S-1), the others byte - 128), slice pairs
+    p + q <= S - 1 summed exactly in int64, folded into FP64 per product in k
+    order."""
     H = sum(128 << (8 * j) for j in range(S - 1))
 
     def digits(x):  # vectors along axis 0
         mx = np.abs(x).max(axis=1)
         e = np.where(mx > 0, np.frexp(mx)[1], 0)
+        lower = (mx > 0) & (np.rint(np.ldexp(mx, 8 * S - 1 - e)).astype(np.int64) + H < (1 << (8 * S - 1)))
+        e = e - lower.astype(e.dtype)
         m = np.rint(np.ldexp(x, (8 * S - 2 - e)[:, None])).astype(np.int64) + H
+        lead = m >> (8 * (S - 1))
+        assert lead.min() >= -128 and lead.max() <= 127
         return [m >> (8 * (S - 1))] + [((m >> (8 * j)) & 255) - 128 for j in range(S - 2, -1, -1)], e
 
     re_, ke, ce = np.cumsum([0] + rsz), np.cumsum([0] + ksz), np.cumsum([0] + csz)
@@ -254,3 +260,29 @@ def test_kwindows_split_precision(monkeypatch):
     assert ctx().kwindows() > 4
     assert rel_fro(got, ref) < TOL
 
+
+
+@pytest.mark.parametrize("slices,stype", [(4, sa.SLICE_FP16), (3, sa.SLICE_FP16), (4, sa.SLICE_BF16),
+                                          (5, sa.SLICE_INT8)])
+def test_band_full_size_error_per_block_row(slices, stype):
+    """The split-precision band workload at full size (n = 2048, 64-blocks,
+    band 4, thresh 1e-10) with 3-4 FP16 / 4 BF16 slices as the workload names
+    and the INT8 5-slice form: task list bit-exact against the FP64 oracle and
+    the relative error per block row stated against a bound set by the digit
+    count (2^-(bits) growth: 3 FP16 slices carry 28 bits, 4 BF16 31)."""
+    cfg = synth.band_workload()
+    dA = sa.BSM.uniform(ctx(), 2048, 2048, 64, 64, cfg.thresh).generate(1, cfg.a_scale)
+    dB = sa.BSM.uniform(ctx(), 2048, 2048, 64, 64, cfg.thresh).generate(2, cfg.b_scale)
+    oA = oracle.OBsm.uniform(2048, 2048, 64, 64, cfg.thresh).from_dense(dA.to_dense())
+    oB = oracle.OBsm.uniform(2048, 2048, 64, 64, cfg.thresh).from_dense(dB.to_dense())
+    oC, dC = empty_pair_uniform(2048, 2048, 64, 64)
+    ref, got = run(oA, dA, 0, oB, dB, 0, oC, dC, cfg.thresh,
+                   sa.default_opts(mode=sa.MODE_OZAKI, slices=slices, slice_type=stype))
+    rows = block_row_errors(got, ref, cfg.rsz)
+    bound = {(4, sa.SLICE_FP16): TOL, (3, sa.SLICE_FP16): 1e-7, (4, sa.SLICE_BF16): 1e-8,
+             (5, sa.SLICE_INT8): TOL}[(slices, stype)]
+    print(f"band n=2048 {slices} slices type {stype}: rel err {rel_fro(got, ref):.3e}, "
+          f"per block row max {rows.max():.3e} median {np.median(rows):.3e}")
+    assert rel_fro(got, ref) < bound
+    assert rows.max() < bound
+    assert len(ctx().tasklist()) == 2292

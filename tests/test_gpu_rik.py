@@ -106,3 +106,47 @@ def test_ri_k_step_vs_oracle(mode):
             Bb = Bp[:, ob:ob + mub * icb].reshape(-1, icb, mub, order="F")
             Kd[mu_off[ma]:mu_off[ma + 1], mu_off[mb]:mu_off[mb + 1]] += np.einsum("pim,pin->mn", Ba, Bb)
     assert rel_fro(K, Kd) < (1e-11 if mode == "fp64" else 1e-10)
+
+
+@pytest.mark.parametrize("kind,s_t,s_k,shared_t,shared_k,win_mb", [
+    ("int8", 5, 5, 0, 0, 1),   # the benchmarked form, K contraction in many k-windows
+    ("int8", 5, 5, 0, 0, 0),   # the same in one launch
+    ("int8", 5, 5, 1, 0, 0),   # shared exponents in the transform
+    ("int8", 5, 5, 0, 1, 1),   # shared exponents on the K contraction's one shared slicing
+    ("bf16", 5, 5, 0, 0, 1),
+    ("fp16", 4, 4, 0, 0, 1),
+])
+def test_ri_k_step_split_forms_vs_oracle(kind, s_t, s_k, shared_t, shared_k, win_mb, monkeypatch):
+    """The split-precision forms of the RI-K step against the FP64 oracle on
+    the same X and B: K task list bit-exact, B' and K within 1e-10.  The K
+    contraction B'^T B' reads one slicing of B' in both roles (a reshape view
+    used transposed and plain); SA_KWINDOW_MB=1 splits it into k-windows."""
+    monkeypatch.setenv("SA_KWINDOW_MB", str(win_mb))
+    st = {"int8": sa.SLICE_INT8, "bf16": sa.SLICE_BF16, "fp16": sa.SLICE_FP16}[kind]
+    w = synth.rik_small()
+    dX = sa.BSM(ctx(), w.p_sizes, w.p_sizes, w.thresh)
+    X, _, _ = sa.inv_sqrt(ctx(), w.metric(), X_bsm=dX, tol=1e-14)
+    dB, oB, oX = build_pair(w, X)
+    dBp = sa.BSM(ctx(), w.p_sizes, w.b_col_sizes, 0.0)
+    dK = sa.BSM(ctx(), w.mu_sizes, w.mu_sizes, 0.0)
+    ot = sa.default_opts(mode=sa.MODE_OZAKI, slices=s_t, slice_type=st, shared_exp=shared_t)
+    ok = sa.default_opts(mode=sa.MODE_OZAKI, slices=s_k, slice_type=st, shared_exp=shared_k)
+    sa.ri_k_step(dX, dB, dBp, dK, w.mu_sizes, w.i_chunks, w.thresh, ot, ok)
+    nwin = ctx().kwindows()
+    t_dev = ctx().tasklist()
+    K_ref, Bp_ref, _, t2 = oracle_rik(w, oX, oB)
+    assert np.array_equal(t2, t_dev), "K task lists differ"
+    if win_mb:
+        assert nwin > 4, f"expected the K contraction in k-windows, got {nwin}"
+    else:
+        assert nwin == 1
+    assert rel_fro(dBp.to_dense(), Bp_ref) < 1e-10
+    K = dK.to_dense()
+    err = rel_fro(K, K_ref)
+    assert err < 1e-10, err
+    # per block row of K, as the split-precision error is reported
+    mo = np.concatenate([[0], np.cumsum(w.mu_sizes)])
+    for r in range(len(w.mu_sizes)):
+        ref_r = K_ref[mo[r]:mo[r + 1]]
+        if np.linalg.norm(ref_r) > 0:
+            assert rel_fro(K[mo[r]:mo[r + 1]], ref_r) < 1e-9

@@ -192,3 +192,54 @@ def test_preconditions_rejected():
     _, dG = pair_uniform(np.ones((60, 60)), 16, 16, 0.0)
     with pytest.raises(sa.SAError):
         sa.matmult_strassen_sparse(dG, dE, dF, 0.0, 0, tree=oracle.forced_tree(4, {1}))
+
+
+def hybrid_shape_integer(nb, bs, band, seed):
+    """Band matrices of the large hybrid workload's shape (128-blocks, band
+    |I-J| <= band) with integer entries in [-4, 4] times power-of-two block
+    scales 2^-(|I-J| // 8): every FP64 operation of every path (screened
+    leaves, Winograd pre/post-adds, dense roots) is exact, so C must equal the
+    dense product bit for bit."""
+    rng = np.random.default_rng(seed)
+    n = nb * bs
+    sz = synth.uniform_sizes(n, bs)
+    I = np.arange(nb)
+    d = np.abs(I[:, None] - I[None, :])
+    sc = np.where(d <= band, 2.0 ** -(d // 8), 0.0)
+    rb = np.repeat(np.arange(nb), sz)
+    a = rng.integers(-4, 5, (n, n)).astype(float) * sc[rb][:, rb]
+    b = rng.integers(-4, 5, (n, n)).astype(float) * sc[rb][:, rb]
+    return n, np.asfortranarray(a), np.asfortranarray(b)
+
+
+@pytest.mark.parametrize("cutoff", [2048, 512])
+def test_hybrid_workload_shape_exact_batched(cutoff, monkeypatch):
+    """The large hybrid recursion at n = 8192 in its own blocking (128-blocks,
+    band 24 of 64 block rows ~ the n = 32768 band 95 of 256), threshold 1e-10:
+    the cost-model tree, the learned plan (committed model) and a forced
+    level-2 tree, each with the Strassen scratch budget capped (SA_SCRATCH_MB)
+    so that every root runs in a batch of its own.  cutoff 2048 is the
+    workload's (min_size_for_strassen); 512 lets the recursion reach depth 4
+    at this size."""
+    monkeypatch.setenv("SA_SCRATCH_MB", "1")
+    nb, bs = 64, 128
+    n, a, b = hybrid_shape_integer(nb, bs, 24, 7 + cutoff)
+    ref = a @ b
+    c = ctx()
+    sa.mlp_load(c, sa.DEFAULT_PATH_MODEL)
+    dA = sa.BSM.uniform(c, n, n, bs, bs, 1e-10).from_dense(a)
+    dB = sa.BSM.uniform(c, n, n, bs, bs, 1e-10).from_dense(b)
+    dC = sa.BSM.uniform(c, n, n, bs, bs, 0.0)
+    most_batches = 0
+    for name, opts, tree in [("heuristic", sa.default_opts(scheduler=sa.SCHED_HEURISTIC), None),
+                             ("learned", sa.default_opts(scheduler=sa.SCHED_MLP), None),
+                             ("forced", None, oracle.forced_tree(nb, {2}))]:
+        dC.fill(0.0)
+        sa.matmult_strassen_sparse(dC, dA, dB, 1e-10, cutoff, opts=opts, tree=tree)
+        st = sa.strassen_last_stats()
+        print(name, cutoff, st, flush=True)
+        assert np.array_equal(dC.to_dense(), ref), f"{name}: hybrid result differs from the exact product"
+        if st["nroots"] > 0:
+            assert st["batches"] == st["nroots"], (name, st)
+        most_batches = max(most_batches, st["batches"])
+    assert most_batches >= 3

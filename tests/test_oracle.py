@@ -478,3 +478,145 @@ def test_mlp_oracle_adam_decreases_loss():
     for t in range(1, 301):
         prm, m1, m2 = mlp.adam(prm, mlp.grad(prm, X, T, M, mu, sd), m1, m2, t, 1e-2)
     assert mlp.loss(prm, X, T, M, mu, sd) < 0.05 * l0
+
+
+# ------------------------------------- recompress / thresholded dot, pinned --
+def test_scale_recompress_designed_ties():
+    """scale() multiplies cached norms by s*s and recompress() drops cached
+    norms below thresh * (target block size) -- fixed by construction, with
+    blocks exactly at the threshold and ragged blocks between the target-size
+    and actual-size thresholds (tests/designed.py)."""
+    import designed
+
+    a, thr, s, pres, norms = designed.recompress_case()
+    m = oracle.OBsm.uniform(designed.NR, designed.NC, designed.BS, designed.BS, thr).from_dense(a)
+    m.scale(s)
+    m.recompress()
+    p, n = m.meta()
+    assert np.array_equal(p, pres)
+    assert np.array_equal(n[p], norms[p])
+    assert np.array_equal(m.to_dense(), np.where(np.abs(a) * s * s >= 1.0, a * s, 0.0))
+
+
+def test_dot_thresholded_designed_ties():
+    import designed
+
+    a, b, thr, expect, expect_all = designed.dot_case()
+    A = oracle.OBsm.uniform(designed.NR, designed.NC, designed.BS, designed.BS, 0.0).from_dense(a)
+    B = oracle.OBsm.uniform(designed.NR, designed.NC, designed.BS, designed.BS, 0.0).from_dense(b)
+    assert A.dot(B, thr) == expect
+    assert A.dot(B, 0.0) == expect_all
+    assert A.dot(B, 0.0) == float(np.sum(a * b))
+
+
+@needs_ref
+def test_scale_recompress_vs_reference():
+    """The oracle's scale + recompress against the reference's templates, on
+    the designed ties and on random ragged matrices at several thresholds."""
+    import designed
+
+    R = oracle.ref()
+    cases = [designed.recompress_case()[:3]]
+    rng = np.random.default_rng(61)
+    for thr, s in [(1e-3, 0.7), (5e-2, 0.3), (0.2, -1.5)]:
+        cases.append((rand_bsm_dense(rng, [5] * 7 + [3], [5] * 5 + [2], 0.7, decay=0.6), thr, s))
+    for a, thr, s in cases:
+        nr, nc = a.shape
+        bs = 4 if (nr, nc) == (designed.NR, designed.NC) else 5
+        m = oracle.OBsm.uniform(nr, nc, bs, bs, thr).from_dense(a)
+        m.scale(s)
+        m.recompress()
+        p, n = m.meta()
+        nb = m.nrb * m.ncb
+        rp = np.zeros(nb, np.uint8); rn = np.zeros(nb); rd = np.zeros((nr, nc), order="F")
+        R.ref_scale_recompress(oracle._ptr(np.asfortranarray(a)), nr, nc, bs, bs, thr, s, oracle._ptr(rp),
+                               oracle._ptr(rn), oracle._ptr(rd))
+        rp = rp.reshape(m.nrb, m.ncb, order="F").astype(bool)
+        rn = rn.reshape(m.nrb, m.ncb, order="F")
+        assert np.array_equal(p, rp)
+        np.testing.assert_allclose(n[p], rn[rp], rtol=4e-15, atol=0)
+        assert np.array_equal(m.to_dense(), rd)
+
+
+@needs_ref
+def test_dot_thresholded_vs_reference():
+    import designed
+
+    R = oracle.ref()
+    a, b, thr, expect, _ = designed.dot_case()
+    got = R.ref_dot(oracle._ptr(np.asfortranarray(a)), oracle._ptr(np.asfortranarray(b)), designed.NR, designed.NC,
+                    designed.BS, designed.BS, 0.0, thr)
+    assert got == expect
+    rng = np.random.default_rng(62)
+    for thr in [0.0, 1e-3, 0.05, 0.3]:
+        a = rand_bsm_dense(rng, [6] * 6 + [4], [6] * 4 + [1], 0.8, decay=0.5)
+        b = rand_bsm_dense(rng, [6] * 6 + [4], [6] * 4 + [1], 0.8, decay=0.5)
+        A = oracle.OBsm.uniform(a.shape[0], a.shape[1], 6, 6, 0.0).from_dense(a)
+        B = oracle.OBsm.uniform(a.shape[0], a.shape[1], 6, 6, 0.0).from_dense(b)
+        want = R.ref_dot(oracle._ptr(a), oracle._ptr(b), a.shape[0], a.shape[1], 6, 6, 0.0, thr)
+        got = A.dot(B, thr)
+        assert abs(got - want) <= 1e-14 * max(1.0, abs(want)), (thr, got, want)
+
+
+def _near_tie_setup():
+    """The designed near-tie operands and a threshold at the median estimate
+    (thresh * bs^3 reproduces it exactly: bs^3 is a power of two)."""
+    import designed
+
+    a, b, bs = designed.near_tie_case()
+    n = a.shape[0]
+    A = oracle.OBsm.uniform(n, n, bs, bs, 0.0).from_dense(a)
+    B = oracle.OBsm.uniform(n, n, bs, bs, 0.0).from_dense(b)
+    _, na = A.meta(); _, nbm = B.meta()
+    est = (na[:, :, None] * nbm[None, :, :]).ravel()
+    thresh = float(np.median(np.sort(est)[: est.size // 2 * 2 - 1])) / float(bs ** 3)
+    return a, b, bs, thresh, A, B, na, nbm
+
+
+def test_near_tie_screening_follows_canonical_norms():
+    """At near-ties the oracle keeps exactly the triples whose canonical-norm
+    product reaches the threshold (one FP64 product and one comparison, from
+    the oracle's cached norms), and both outcomes occur."""
+    a, b, bs, thresh, A, B, na, nbm = _near_tie_setup()
+    nb = A.nrb
+    Cm = oracle.OBsm.uniform(a.shape[0], a.shape[0], bs, bs, 0.0)
+    t = oracle.matmult(Cm, A, 0, B, 0, thresh)
+    thr = thresh * float(bs * bs * bs)
+    keep = [(i, j, k) for j in range(nb) for i in range(nb) for k in range(nb) if na[i, k] * nbm[k, j] >= thr]
+    assert np.array_equal(t, np.array(keep, dtype=np.int32).reshape(-1, 3))
+    assert 0 < len(t) < nb ** 3
+    est = na[:, :, None] * nbm[None, :, :]
+    assert np.abs(est / thr - 1).max() < 1e-14
+
+
+@needs_ref
+def test_near_tie_screening_vs_reference():
+    """The reference's screened set at near-ties: its block norms come from
+    MKL dnrm2, whose summation order differs from the canonical one, so C
+    blocks may differ -- each such block holds a triple within 1e-14 of the
+    threshold (all triples of this case are).  Away from ties the task lists
+    match (test_matmult_vs_reference)."""
+    a, b, bs, thresh, A, B, na, nbm = _near_tie_setup()
+    n = a.shape[0]
+    nb = n // bs
+    R = oracle.ref()
+    c = np.zeros((n, n), order="F")
+    R.ref_matmult(oracle._ptr(a), n, n, 0.0, oracle._ptr(b), n, n, 0.0, oracle._ptr(c), n, n, 0.0, 1, bs, 0, 0,
+                  thresh, 1.0, 0.0, None, None, None)
+    Cm = oracle.OBsm.uniform(n, n, bs, bs, 0.0)
+    t = oracle.matmult(Cm, A, 0, B, 0, thresh)
+    co = Cm.to_dense()
+    differ = 0
+    for j in range(nb):
+        for i in range(nb):
+            blk = (slice(i * bs, (i + 1) * bs), slice(j * bs, (j + 1) * bs))
+            if np.abs(co[blk] - c[blk]).max() > 1e-12 * max(1.0, np.abs(c[blk]).max()):
+                differ += 1
+    print(f"near-tie case: {len(t)} of {nb ** 3} triples kept by the canonical norms; "
+          f"{differ} of {nb * nb} C blocks differ from the reference (MKL dnrm2 norms)")
+    # MKL's norms of the same multiset of values: equal to the canonical ones
+    # to within 2 ulp, so the reference keeps every triple whose estimate is
+    # clearly above, drops every one clearly below
+    rn = np.zeros(nb * nb); rp = np.zeros(nb * nb, np.uint8)
+    R.ref_from_dense(oracle._ptr(a), n, n, bs, bs, 0.0, oracle._ptr(rp), oracle._ptr(rn))
+    np.testing.assert_allclose(rn.reshape(nb, nb, order="F"), na, rtol=4.5e-16, atol=0)
