@@ -66,10 +66,38 @@ void* pinned(sa_context* c, size_t bytes) {
   return c->pinned;
 }
 
+void* TableMem::raw(const char* name, size_t bytes) {
+  if (!fixed()) {
+    DevBuf& b = c->buf(name);
+    b.reserve(bytes);
+    return b.p;
+  }
+  const size_t off = (used + 255) & ~(size_t)255;
+  if (off > cap || bytes > cap - off) {
+    set_error("allocation-free multiply: table '%s' (%zu bytes) does not fit the %zu-byte scratch region (%zu in use)",
+              name, bytes, cap, off);
+    throw Error{SA_ERR_INVALID};
+  }
+  used = off + bytes;
+  return base + off;
+}
+
+void TableMem::read(const int64_t* d, int64_t* h, int n) {
+  if (fixed()) {
+    SA_CHECK_HIP(hipMemcpyAsync(h, d, sizeof(int64_t) * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+    SA_CHECK_HIP(hipStreamSynchronize(c->stream));
+    return;
+  }
+  int64_t* p = static_cast<int64_t*>(pinned(c, sizeof(int64_t) * (size_t)n));
+  SA_CHECK_HIP(hipMemcpyAsync(p, d, sizeof(int64_t) * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+  SA_CHECK_HIP(hipStreamSynchronize(c->stream));
+  for (int i = 0; i < n; ++i) h[i] = p[i];
+}
+
 void prof_begin(sa_context* c, const char* cls, hipEvent_t* ev0) {
   (void)cls;
   *ev0 = nullptr;
-  if (!c->profiling) return;
+  if (!c->profiling || c->quiet) return;
   SA_CHECK_HIP(hipEventCreate(ev0));
   SA_CHECK_HIP(hipEventRecord(*ev0, c->stream));
 }
@@ -165,14 +193,21 @@ static void scan_rec(sa_context* c, const int64_t* src, int64_t* d, int64_t n, i
                      (const int64_t*)part);
 }
 
-void scan_exclusive_i64_async(sa_context* c, const int64_t* src, int64_t* d, int64_t n, int64_t* total_dev) {
+int64_t scan_ws_elems(int64_t n) {
   int64_t need = 2, m = n;
   while (m > 1) {
     m = ceil_div(m, kScanTile);
     need += m;
   }
-  c->ws_scan.reserve(sizeof(int64_t) * (size_t)need);
-  int64_t* ws = c->ws_scan.as<int64_t>();
+  return need;
+}
+
+void scan_exclusive_i64_async(sa_context* c, const int64_t* src, int64_t* d, int64_t n, int64_t* total_dev,
+                              int64_t* ws) {
+  if (!ws) {
+    c->ws_scan.reserve(sizeof(int64_t) * (size_t)scan_ws_elems(n));
+    ws = c->ws_scan.as<int64_t>();
+  }
   if (n == 0) {
     fill_i64(c, total_dev, 1, 0);
   } else {
@@ -186,11 +221,12 @@ void scan_exclusive_i64_async(sa_context* c, int64_t* d, int64_t n, int64_t* tot
 }
 
 void scan3_exclusive_i64_async(sa_context* c, const int64_t* const src[3], int64_t* const d[3], const int64_t n[3],
-                               int64_t* const total[3]) {
+                               int64_t* const total[3], int64_t* ws) {
   bool small = true;
   for (int i = 0; i < 3; ++i) small = small && n[i] >= 1 && n[i] <= kScanTile;
   if (!small) {
-    for (int i = 0; i < 3; ++i) scan_exclusive_i64_async(c, src[i], d[i], n[i], total[i]);
+    // one after the other on the stream: they may share one workspace
+    for (int i = 0; i < 3; ++i) scan_exclusive_i64_async(c, src[i], d[i], n[i], total[i], ws);
     return;
   }
   ScanJobs j;
@@ -202,6 +238,15 @@ void scan3_exclusive_i64_async(sa_context* c, const int64_t* const src[3], int64
   }
   hipLaunchKernelGGL(k_scan_jobs, dim3(3), dim3(kScanThreads), 0, c->stream, j);
   SA_CHECK_HIP(hipGetLastError());
+}
+
+int64_t scan_exclusive_i64(TableMem& mem, int64_t* d, int64_t n) {
+  const int64_t need = scan_ws_elems(n);
+  int64_t* ws = mem.get<int64_t>("scan_ws", need);
+  scan_exclusive_i64_async(mem.c, d, d, n, ws + need - 1, ws);
+  int64_t t = 0;
+  mem.read(ws + need - 1, &t, 1);
+  return t;
 }
 
 int64_t scan_exclusive_i64(sa_context* c, int64_t* d, int64_t n, bool read_total) {

@@ -534,6 +534,32 @@ static int autotune(sa_context* c, const GemmBatch& b) {
   return best;
 }
 
+GemmPlan gemm_fp64_plan_fixed(const GemmBatch& b, int cfg, TableMem& mem) {
+  if (b.nouts == 0) return GemmPlan{};
+  if (cfg < 0 || cfg >= kNumCfgs) {
+    cfg = heuristic_cfg(b);
+    std::lock_guard<std::mutex> lk(tune_mutex());
+    auto it = tune_cache().find(std::make_tuple(b.ta, b.tb, b.max_m, b.max_n, b.max_k));
+    if (it != tune_cache().end()) cfg = it->second;
+  }
+  const Cfg& k = kCfgs[cfg];
+  const GemmItem* items = nullptr;
+  int64_t nitems = b.nouts;
+  if (b.max_m > k.tm || b.max_n > k.tn) {
+    sa_context* c = mem.c;
+    int64_t* cnt = mem.get<int64_t>("gi_cnt", b.nouts + 1);
+    hipLaunchKernelGGL(k_count_tiles, dim3((unsigned)ceil_div(b.nouts, 256)), dim3(256), 0, c->stream, b.outs, b.nouts,
+                       k.tm, k.tn, cnt);
+    nitems = scan_exclusive_i64(mem, cnt, b.nouts);
+    SA_REQUIRE(nitems < (int64_t)1 << 31, "too many GEMM work items");
+    GemmItem* it = mem.get<GemmItem>("gi_items", nitems + 1);
+    hipLaunchKernelGGL(k_emit_items, dim3((unsigned)ceil_div(b.nouts, 256)), dim3(256), 0, c->stream, b.outs,
+                       b.nouts, k.tm, k.tn, (const int64_t*)cnt, it);
+    items = it;
+  }
+  return GemmPlan{cfg, items, nitems};
+}
+
 GemmPlan gemm_fp64_plan(sa_context* c, const GemmBatch& b, int cfg) {
   if (b.nouts == 0) return GemmPlan{};
   if (cfg < 0 || cfg >= kNumCfgs) cfg = autotune(c, b);

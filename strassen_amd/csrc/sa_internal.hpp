@@ -81,6 +81,29 @@ struct DevBuf {
 
 __host__ __device__ inline int64_t ceil_div(int64_t a, int64_t b) { return a == 0 ? 0 : 1 + (a - 1) / b; }
 
+// Device memory for the tables of one multiply (task lists, records, work
+// items, scan partials).  By default these are the context's named grow-only
+// buffers; with a fixed region (the allocation-free entry point) they are
+// carved from it by a bump pointer in 256-byte steps -- nothing is allocated,
+// and a table that does not fit raises SA_ERR_INVALID before any kernel
+// touches it.
+struct TableMem {
+  sa_context* c = nullptr;
+  uint8_t* base = nullptr;  // fixed region; nullptr = context buffers
+  size_t cap = 0, used = 0;
+  explicit TableMem(sa_context* ctx) : c(ctx) {}
+  TableMem(sa_context* ctx, void* region, size_t bytes) : c(ctx), base(static_cast<uint8_t*>(region)), cap(bytes) {}
+  bool fixed() const { return base != nullptr; }
+  void* raw(const char* name, size_t bytes);
+  template <class T>
+  T* get(const char* name, int64_t n) {
+    return static_cast<T*>(raw(name, sizeof(T) * (size_t)(n > 0 ? n : 1)));
+  }
+  // n int64 values device -> host, then wait for the stream (pinned staging
+  // for context buffers; straight into h for a fixed region)
+  void read(const int64_t* d, int64_t* h, int n);
+};
+
 // Work-item index of workgroup `bid` such that consecutive items run on the
 // same XCD: workgroups are dispatched round-robin over the 8 XCDs, so item
 // ranges of about n/8 land on one XCD and share its L2.
@@ -156,6 +179,11 @@ struct sa_context {
   // start ("mm_seg") and the kb list ("mm_kb")
   int64_t last_ntasks = 0, last_nc = 0;
   int32_t last_nib = 0;
+  const int64_t* last_cnt = nullptr;  // device tables of that task list
+  const int64_t* last_seg = nullptr;
+  const int32_t* last_kb = nullptr;
+  bool last_sym = false;  // blocks ib > jb hold the mirrored task lists of (jb, ib)
+  bool quiet = false;     // no profiling events (allocation-free calls)
   int32_t last_gemm_cfg = -1;
   int32_t last_kwindows = 1;  // k-windows of the last screened multiply
   // learned path selector (device copy of the network in buffer "mlp_prm")
@@ -214,14 +242,20 @@ struct ProfScope {
 
 // exclusive scan of n int64 values in place (device), returns the total (sync)
 int64_t scan_exclusive_i64(sa_context* c, int64_t* d, int64_t n, bool read_total = true);
-// same, total written to a device slot, no synchronisation
+// the same with its partial sums in mem
+int64_t scan_exclusive_i64(TableMem& mem, int64_t* d, int64_t n);
+// int64 workspace elements a scan of n values needs
+int64_t scan_ws_elems(int64_t n);
+// same, total written to a device slot, no synchronisation; ws (scan_ws_elems
+// elements) or the context's scan workspace when null
 void scan_exclusive_i64_async(sa_context* c, int64_t* d, int64_t n, int64_t* total_dev);
 // the same from a separate input array (left untouched) into d
-void scan_exclusive_i64_async(sa_context* c, const int64_t* src, int64_t* d, int64_t n, int64_t* total_dev);
+void scan_exclusive_i64_async(sa_context* c, const int64_t* src, int64_t* d, int64_t n, int64_t* total_dev,
+                              int64_t* ws = nullptr);
 // three independent scans (src[i] -> d[i], n[i] values, total[i]); one launch
 // when each fits one scan tile (2048 values)
 void scan3_exclusive_i64_async(sa_context* c, const int64_t* const src[3], int64_t* const d[3], const int64_t n[3],
-                               int64_t* const total[3]);
+                               int64_t* const total[3], int64_t* ws = nullptr);
 // read one int64 from device (sync)
 int64_t read_i64(sa_context* c, const int64_t* d);
 
@@ -283,11 +317,23 @@ struct ScreenResult {
   const int64_t* need_scan = nullptr;
   const int64_t* out_scan = nullptr;  // indexed by out_slot(ib, jb, nib)
   const int32_t* kb = nullptr;
+  bool sym = false;
 };
+// sym: C = op(A) op(B) is symmetric (op(B) = op(A)^T, the RI-K K
+// contraction): only C blocks ib <= jb are screened and computed, and
+// mirror_lower() then writes each block (jb, ib) as the transpose of (ib, jb);
+// the task list of a mirrored block is its partner's, triple for triple.
+ScreenResult screen_and_allocate(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, double thresh,
+                                 const uint8_t* d_tree, const int64_t* d_lvl_off, int nlev, bool c_must_exist,
+                                 TableMem& mem, bool sym = false);
 ScreenResult screen_and_allocate(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, double thresh,
                                  const uint8_t* d_tree, const int64_t* d_lvl_off, int nlev, bool c_must_exist);
+void mirror_lower(sa_bsm* C, const ScreenResult& R);
 int64_t ozaki_multiply(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, double thresh, double alpha,
-                       const sa_opts_t& o);
+                       const sa_opts_t& o, bool sym = false);
+int64_t screened_multiply(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, double thresh, double alpha,
+                          int tile_cfg, const uint8_t* d_tree, const int64_t* d_lvl_off, int nlev, bool c_must_exist,
+                          bool sym = false);
 // k-window batching.  When the operands of a multiply far exceed the
 // memory-side cache (Infinity Cache, 256 MB) while the C blocks it writes fit
 // in it, the products are split by inner block index kb into windows of
@@ -328,6 +374,9 @@ struct GemmPlan {
   int64_t nitems = 0;
 };
 GemmPlan gemm_fp64_plan(sa_context* c, const GemmBatch& b, int cfg);
+// the same without timing runs or allocation: cfg, else the autotuner's
+// cached choice for this shape class, else the size heuristic; work items in mem
+GemmPlan gemm_fp64_plan_fixed(const GemmBatch& b, int cfg, TableMem& mem);
 void gemm_fp64_launch(sa_context* c, const GemmBatch& b, const GemmPlan& g);
 // run f's launches on c's stream as one HIP graph (captured, instantiated,
 // launched once); f must not synchronise or copy from the host

@@ -16,9 +16,10 @@
 //  6. one grouped FP64 MFMA launch over all output tiles.
 // A C block created here starts with a zero cached norm, like the reference's
 // Mat(ni, nj, alloc) at blocksparsematrix.hpp:398-404.
-#include <cstring>
-
 #include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <vector>
 
 #include "sa_internal.hpp"
 
@@ -45,6 +46,9 @@ struct ScreenArgs {
   // optional: products into an empty C block are dropped (Strassen entry,
   // strassen.hpp:441-444 -- the leaf dgemm runs with m=n=0)
   const int64_t* coff;
+  // symmetric C: blocks below the diagonal are not screened (their partners
+  // above it carry the same triples)
+  int32_t sym;
 };
 
 __device__ inline bool covered_by_strassen(const ScreenArgs& s, int ib, int jb, int kb) {
@@ -69,7 +73,7 @@ __global__ void __launch_bounds__(256) k_screen(ScreenArgs s, int64_t* __restric
   const int64_t sij = (int64_t)s.iscr[ib] * (int64_t)s.jscr[jb];
   int64_t pos = PASS == 2 ? cnt_or_seg[c] : 0;
   const int r0 = s.rowstart[ib];
-  const int r1 = (s.coff && s.coff[c] < 0) ? r0 : s.rowstart[ib + 1];
+  const int r1 = ((s.coff && s.coff[c] < 0) || (s.sym && ib > jb)) ? r0 : s.rowstart[ib + 1];
   for (int q0 = r0; q0 < r1; q0 += 64) {
     const int q = q0 + lane;
     bool ok = false;
@@ -196,7 +200,7 @@ __global__ void k_build_records(MMTables t, GemmOut* __restrict__ outs, GemmProd
 // slot's flag, 0 for padding, and for a real C block its arena need
 __global__ void k_need_and_flags(const int64_t* __restrict__ cnt, const int64_t* __restrict__ coff,
                                  const int32_t* __restrict__ crsz, const int32_t* __restrict__ ccsz, int32_t nib,
-                                 int32_t njb, int64_t nslots, int64_t* __restrict__ need,
+                                 int32_t njb, int64_t nslots, int sym, int64_t* __restrict__ need,
                                  int64_t* __restrict__ outflag) {
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nslots) return;
@@ -209,8 +213,43 @@ __global__ void k_need_and_flags(const int64_t* __restrict__ cnt, const int64_t*
   }
   const int64_t c = (int64_t)ib + (int64_t)jb * nib;
   const bool any = cnt[c] > 0;
-  need[c] = (any && coff[c] < 0) ? (int64_t)crsz[ib] * ccsz[jb] : 0;
+  // a mirrored block (sym, below the diagonal) is written, not computed
+  const bool mirrored = sym && ib > jb && cnt[(int64_t)jb + (int64_t)ib * nib] > 0;
+  need[c] = ((any || mirrored) && coff[c] < 0) ? (int64_t)crsz[ib] * ccsz[jb] : 0;
   outflag[s] = any ? 1 : 0;
+}
+
+// Symmetric C: block (ib, jb), ib > jb, becomes the transpose of (jb, ib)
+// wherever that block received products; a block created here starts with a
+// zero cached norm, as every block a multiply creates.  One workgroup per
+// lower block; the mirrored block's arena offset was assigned with the rest.
+__global__ void __launch_bounds__(256) k_mirror_lower(const int64_t* __restrict__ cnt, int64_t* __restrict__ coff,
+                                                      double* __restrict__ cnorm, double* __restrict__ carena,
+                                                      const int32_t* __restrict__ crsz, const int64_t* __restrict__ need,
+                                                      int64_t cbase, int32_t nib) {
+  const int64_t c = blockIdx.x;
+  const int ib = (int)(c % nib), jb = (int)(c / nib);
+  if (ib <= jb) return;
+  const int64_t up = (int64_t)jb + (int64_t)ib * nib;
+  if (cnt[up] == 0) return;
+  __shared__ int64_t dst;
+  if (threadIdx.x == 0) {
+    int64_t o = coff[c];
+    if (o < 0) {
+      o = cbase + need[c];
+      coff[c] = o;
+      cnorm[c] = 0.0;
+    }
+    dst = o;
+  }
+  __syncthreads();
+  const int m = crsz[ib], n = crsz[jb];  // block (ib, jb) is m x n; its partner n x m
+  const double* src = carena + coff[up];
+  double* d = carena + dst;
+  for (int64_t e = threadIdx.x; e < (int64_t)m * n; e += blockDim.x) {
+    const int r = (int)(e % m), q = (int)(e / m);
+    d[e] = src[q + (int64_t)r * n];
+  }
 }
 
 // geometry of op(X) for screening / checking
@@ -249,55 +288,53 @@ void check_mm_geometry(const sa_bsm* C, const sa_bsm* A, bool ta, const sa_bsm* 
 // exclude triples covered by executed Strassen nodes.
 ScreenResult screen_and_allocate(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, double thresh,
                                  const uint8_t* d_tree, const int64_t* d_lvl_off, int nlev, bool c_must_exist) {
+  TableMem mem(C->ctx);
+  return screen_and_allocate(C, A, ta, B, tb, thresh, d_tree, d_lvl_off, nlev, c_must_exist, mem);
+}
+
+ScreenResult screen_and_allocate(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, double thresh,
+                                 const uint8_t* d_tree, const int64_t* d_lvl_off, int nlev, bool c_must_exist,
+                                 TableMem& mem, bool sym) {
   sa_context* c = C->ctx;
   const OpGeo ga = op_geo(A, ta), gb = op_geo(B, tb);
   const int32_t nib = ga.nrb, njb = gb.ncb, nkb = ga.ncb;
   const int64_t nc = (int64_t)nib * njb;
   ScreenResult R;
   R.nib = nib; R.njb = njb; R.nkb = nkb; R.nc = nc;
+  R.sym = sym;
   c->last_nc = nc;
   c->last_nib = nib;
   c->last_ntasks = 0;
+  c->last_sym = sym;
   if (nc == 0) return R;
+  SA_REQUIRE(!sym || (nib == njb && C->rsz == C->csz), "matmult: a symmetric product needs a square block grid");
 
-  DevBuf& bcnt = c->buf("mm_cnt");
-  DevBuf& bseg = c->buf("mm_seg");
-  DevBuf& bneed = c->buf("mm_need");
-  DevBuf& bout = c->buf("mm_out");
-  DevBuf& btot = c->buf("mm_totals");
-  bcnt.reserve(sizeof(int64_t) * (size_t)(nc + 1));
-  bseg.reserve(sizeof(int64_t) * (size_t)(nc + 1));
-  bneed.reserve(sizeof(int64_t) * (size_t)(nc + 1));
   const int64_t nslots = out_slots(nib, njb);
-  bout.reserve(sizeof(int64_t) * (size_t)(nslots + 1));
-  btot.reserve(sizeof(int64_t) * 8);
-  int64_t* cnt = bcnt.as<int64_t>();
-  int64_t* seg = bseg.as<int64_t>();
-  int64_t* need = bneed.as<int64_t>();
-  int64_t* outf = bout.as<int64_t>();
-  int64_t* tot = btot.as<int64_t>();
+  int64_t* cnt = mem.get<int64_t>("mm_cnt", nc + 1);
+  int64_t* seg = mem.get<int64_t>("mm_seg", nc + 1);
+  int64_t* need = mem.get<int64_t>("mm_need", nc + 1);
+  int64_t* outf = mem.get<int64_t>("mm_out", nslots + 1);
+  int64_t* tot = mem.get<int64_t>("mm_totals", 8);
+  // scan partials, shared by the scans below (they run in stream order)
+  int64_t* sws = mem.fixed() ? mem.get<int64_t>("mm_scan_ws", scan_ws_elems(std::max<int64_t>(nc, nslots) + 1))
+                             : nullptr;
 
   // CSR of op(A) rows (one extra slot so start[nib] is the total)
-  DevBuf& brc = c->buf("mm_rowcnt");
-  DevBuf& brs = c->buf("mm_rowstart");
-  DevBuf& brl = c->buf("mm_rowlist");
-  brc.reserve(sizeof(int64_t) * (size_t)(nib + 2));
-  brs.reserve(sizeof(int32_t) * (size_t)(nib + 2));
-  brl.reserve(sizeof(int32_t) * (size_t)((int64_t)nib * nkb + 1));
+  int64_t* rc = mem.get<int64_t>("mm_rowcnt", nib + 2);
+  int32_t* rstart = mem.get<int32_t>("mm_rowstart", nib + 2);
+  int32_t* rlist = mem.get<int32_t>("mm_rowlist", (int64_t)nib * nkb + 1);
   {
     const int64_t sr = ta ? A->nrb : 1, sk = ta ? 1 : A->nrb;
-    int64_t* rc = brc.as<int64_t>();
     hipLaunchKernelGGL(k_row_count, dim3((unsigned)ceil_div(nib, 4)), dim3(256), 0, c->stream,
                        (const int64_t*)A->d_off, nib, nkb, sr, sk, rc);
-    scan_exclusive_i64_async(c, rc, nib + 1, tot + 3);
+    scan_exclusive_i64_async(c, rc, rc, nib + 1, tot + 3, sws);
     hipLaunchKernelGGL(k_row_fill, dim3((unsigned)ceil_div(nib + 1, 4)), dim3(256), 0, c->stream,
-                       (const int64_t*)A->d_off, nib, nkb, sr, sk, (const int64_t*)rc, brs.as<int32_t>(),
-                       brl.as<int32_t>());
+                       (const int64_t*)A->d_off, nib, nkb, sr, sk, (const int64_t*)rc, rstart, rlist);
   }
 
   ScreenArgs s;
-  s.rowstart = brs.as<int32_t>();
-  s.rowlist = brl.as<int32_t>();
+  s.rowstart = rstart;
+  s.rowlist = rlist;
   s.aoff = A->d_off; s.anorm = A->d_norm; s.boff = B->d_off; s.bnorm = B->d_norm;
   s.iscr = ga.d_rscr; s.jscr = gb.d_cscr; s.kscr = ga.d_cscr;
   s.a_nrb = A->nrb; s.b_nrb = B->nrb;
@@ -306,6 +343,7 @@ ScreenResult screen_and_allocate(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool 
   s.thresh = thresh;
   s.tree = d_tree; s.lvl_off = d_lvl_off; s.nlev = nlev;
   s.coff = c_must_exist ? C->d_off : nullptr;
+  s.sym = sym ? 1 : 0;
 
   {
     ProfScope ps(c, "screen");
@@ -314,37 +352,53 @@ ScreenResult screen_and_allocate(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool 
   }
   hipLaunchKernelGGL(k_need_and_flags, dim3((unsigned)ceil_div(nslots, 256)), dim3(256), 0, c->stream,
                      (const int64_t*)cnt, (const int64_t*)C->d_off, (const int32_t*)C->d_rsz,
-                     (const int32_t*)C->d_csz, nib, njb, nslots, need, outf);
+                     (const int32_t*)C->d_csz, nib, njb, nslots, sym ? 1 : 0, need, outf);
   {  // seg = exclusive scan of cnt; need and outf in place
     const int64_t* src[3] = {cnt, need, outf};
     int64_t* const dst[3] = {seg, need, outf};
     const int64_t n[3] = {nc, nc, nslots};
     int64_t* const totals[3] = {tot + 0, tot + 1, tot + 2};
-    scan3_exclusive_i64_async(c, src, dst, n, totals);
+    scan3_exclusive_i64_async(c, src, dst, n, totals, sws);
   }
-  int64_t* h = static_cast<int64_t*>(pinned(c, sizeof(int64_t) * 3));
-  SA_CHECK_HIP(hipMemcpyAsync(h, tot, sizeof(int64_t) * 3, hipMemcpyDeviceToHost, c->stream));
-  SA_CHECK_HIP(hipStreamSynchronize(c->stream));
+  int64_t h[3];
+  mem.read(tot, h, 3);
   R.ntasks = h[0];
   R.nnew = h[1];
   R.nouts = h[2];
   c->last_ntasks = R.ntasks;
+  c->last_cnt = cnt;
+  c->last_seg = seg;
+  c->last_kb = nullptr;
   SA_REQUIRE(R.ntasks < ((int64_t)1 << 31), "matmult: task list exceeds 2^31 entries");
 
   R.cbase = C->used;
-  bsm_ensure_capacity(C, R.cbase + R.nnew);
+  if (mem.fixed()) {
+    SA_REQUIRE((int64_t)(C->arena.bytes / sizeof(double)) >= R.cbase + R.nnew,
+               "allocation-free multiply: C needs %lld more elements of block storage than its arena holds "
+               "(reserve it first with sa_bsm_reserve)",
+               (long long)(R.cbase + R.nnew - (int64_t)(C->arena.bytes / sizeof(double))));
+  } else {
+    bsm_ensure_capacity(C, R.cbase + R.nnew);
+  }
   C->used = R.cbase + R.nnew;
+  R.cnt = cnt; R.seg = seg; R.need_scan = need; R.out_scan = outf;
   if (R.ntasks == 0) return R;
 
-  DevBuf& bkb = c->buf("mm_kb");
-  bkb.reserve(sizeof(int32_t) * (size_t)(R.ntasks + 1));
+  int32_t* kb = mem.get<int32_t>("mm_kb", R.ntasks + 1);
   {
     ProfScope ps(c, "screen");
-    hipLaunchKernelGGL(k_screen<2>, dim3((unsigned)ceil_div(nc, 4)), dim3(256), 0, c->stream, s, seg,
-                       bkb.as<int32_t>());
+    hipLaunchKernelGGL(k_screen<2>, dim3((unsigned)ceil_div(nc, 4)), dim3(256), 0, c->stream, s, seg, kb);
   }
-  R.cnt = cnt; R.seg = seg; R.need_scan = need; R.out_scan = outf; R.kb = bkb.as<int32_t>();
+  R.kb = kb;
+  c->last_kb = kb;
   return R;
+}
+
+void mirror_lower(sa_bsm* C, const ScreenResult& R) {
+  if (!R.sym || R.nc == 0 || R.ntasks == 0) return;
+  hipLaunchKernelGGL(k_mirror_lower, dim3((unsigned)R.nc), dim3(256), 0, C->ctx->stream, R.cnt, C->d_off, C->d_norm,
+                     C->data(), (const int32_t*)C->d_rsz, R.need_scan, R.cbase, R.nib);
+  SA_CHECK_HIP(hipGetLastError());
 }
 
 MMTables mm_tables(const ScreenResult& R, sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb) {
@@ -359,25 +413,23 @@ MMTables mm_tables(const ScreenResult& R, sa_bsm* C, sa_bsm* A, bool ta, sa_bsm*
   return t;
 }
 
-int64_t screened_multiply(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, double thresh, double alpha,
-                          int tile_cfg, const uint8_t* d_tree, const int64_t* d_lvl_off, int nlev,
-                          bool c_must_exist) {
+static int64_t screened_multiply_mem(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, double thresh,
+                                     double alpha, int tile_cfg, const uint8_t* d_tree, const int64_t* d_lvl_off,
+                                     int nlev, bool c_must_exist, bool sym, TableMem& mem) {
   sa_context* c = C->ctx;
-  const ScreenResult R = screen_and_allocate(C, A, ta, B, tb, thresh, d_tree, d_lvl_off, nlev, c_must_exist);
+  const ScreenResult R =
+      screen_and_allocate(C, A, ta, B, tb, thresh, d_tree, d_lvl_off, nlev, c_must_exist, mem, sym);
   if (R.ntasks == 0) return 0;
-  DevBuf& bouts = c->buf("mm_outs");
-  DevBuf& bprods = c->buf("mm_prods");
-  bouts.reserve(sizeof(GemmOut) * (size_t)(R.nouts + 1));
-  bprods.reserve(sizeof(GemmProd) * (size_t)(R.ntasks + 1));
+  GemmOut* outs = mem.get<GemmOut>("mm_outs", R.nouts + 1);
+  GemmProd* prods = mem.get<GemmProd>("mm_prods", R.ntasks + 1);
   const MMTables t = mm_tables(R, C, A, ta, B, tb);
-  hipLaunchKernelGGL(k_build_records, dim3((unsigned)ceil_div(R.nc, 128)), dim3(128), 0, c->stream, t,
-                     bouts.as<GemmOut>(), bprods.as<GemmProd>());
+  hipLaunchKernelGGL(k_build_records, dim3((unsigned)ceil_div(R.nc, 128)), dim3(128), 0, c->stream, t, outs, prods);
 
   const OpGeo ga = op_geo(A, ta);
   GemmBatch gb2;
-  gb2.outs = bouts.as<GemmOut>();
+  gb2.outs = outs;
   gb2.nouts = R.nouts;
-  gb2.prods = bprods.as<GemmProd>();
+  gb2.prods = prods;
   gb2.nprods = R.ntasks;
   gb2.ta = ta; gb2.tb = tb;
   gb2.alpha = alpha;
@@ -386,15 +438,31 @@ int64_t screened_multiply(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, dou
   for (int32_t v : C->csz) mn = std::max(mn, v);
   for (int32_t v : *ga.csz) mk = std::max(mk, v);
   gb2.max_m = mm; gb2.max_n = mn; gb2.max_k = mk;
-  const KWindows W = plan_kwindows(c, A, ta, B, tb, R, bouts.as<GemmOut>());
-  c->last_kwindows = W.nwin;
-  for (int w = 0; w < W.nwin; ++w) {
-    GemmBatch gw = gb2;
-    gw.outs = W.outs + W.off[(size_t)w];
-    gw.nouts = W.off[(size_t)w + 1] - W.off[(size_t)w];
-    if (gw.nouts > 0) gemm_fp64_run(c, gw, tile_cfg);
+  if (mem.fixed()) {  // one launch, tile choice without timing runs, items in the region
+    c->last_kwindows = 1;
+    const GemmPlan g = gemm_fp64_plan_fixed(gb2, tile_cfg, mem);
+    c->last_gemm_cfg = g.cfg;
+    gemm_fp64_launch(c, gb2, g);
+  } else {
+    const KWindows W = plan_kwindows(c, A, ta, B, tb, R, outs);
+    c->last_kwindows = W.nwin;
+    for (int w = 0; w < W.nwin; ++w) {
+      GemmBatch gw = gb2;
+      gw.outs = W.outs + W.off[(size_t)w];
+      gw.nouts = W.off[(size_t)w + 1] - W.off[(size_t)w];
+      if (gw.nouts > 0) gemm_fp64_run(c, gw, tile_cfg);
+    }
   }
+  mirror_lower(C, R);
   return R.ntasks;
+}
+
+int64_t screened_multiply(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, double thresh, double alpha,
+                          int tile_cfg, const uint8_t* d_tree, const int64_t* d_lvl_off, int nlev, bool c_must_exist,
+                          bool sym) {
+  TableMem mem(C->ctx);
+  return screened_multiply_mem(C, A, ta, B, tb, thresh, alpha, tile_cfg, d_tree, d_lvl_off, nlev, c_must_exist, sym,
+                               mem);
 }
 
 // ------------------------------------------------------------- k-windows --
@@ -568,21 +636,39 @@ int sa_tasklist_get(sa_ctx_t c, int32_t* triples, int64_t cap, int64_t* n) {
   return guard([&] {
     SA_REQUIRE(c && n, "null argument");
     *n = c->last_ntasks;
-    if (!triples || cap <= 0 || c->last_ntasks == 0) return;
+    if (c->last_ntasks == 0) return;
+    if (c->last_sym) {  // full count: the mirrored blocks' lists as well
+      std::vector<int64_t> cnt0((size_t)c->last_nc);
+      SA_CHECK_HIP(hipMemcpyAsync(cnt0.data(), c->last_cnt, sizeof(int64_t) * c->last_nc, hipMemcpyDeviceToHost,
+                                  c->stream));
+      SA_CHECK_HIP(hipStreamSynchronize(c->stream));
+      int64_t full = 0;
+      for (int64_t ci = 0; ci < c->last_nc; ++ci) {
+        const int64_t ib = ci % c->last_nib, jb = ci / c->last_nib;
+        full += cnt0[(size_t)(ib > jb ? jb + ib * c->last_nib : ci)];
+      }
+      *n = full;
+    }
+    if (!triples || cap <= 0) return;
     const int64_t nc = c->last_nc, nt = c->last_ntasks;
     std::vector<int64_t> cnt(nc), seg(nc);
     std::vector<int32_t> kb(nt);
-    SA_CHECK_HIP(hipMemcpyAsync(cnt.data(), c->buf("mm_cnt").p, sizeof(int64_t) * nc, hipMemcpyDeviceToHost, c->stream));
-    SA_CHECK_HIP(hipMemcpyAsync(seg.data(), c->buf("mm_seg").p, sizeof(int64_t) * nc, hipMemcpyDeviceToHost, c->stream));
-    SA_CHECK_HIP(hipMemcpyAsync(kb.data(), c->buf("mm_kb").p, sizeof(int32_t) * nt, hipMemcpyDeviceToHost, c->stream));
+    SA_CHECK_HIP(hipMemcpyAsync(cnt.data(), c->last_cnt, sizeof(int64_t) * nc, hipMemcpyDeviceToHost, c->stream));
+    SA_CHECK_HIP(hipMemcpyAsync(seg.data(), c->last_seg, sizeof(int64_t) * nc, hipMemcpyDeviceToHost, c->stream));
+    SA_CHECK_HIP(hipMemcpyAsync(kb.data(), c->last_kb, sizeof(int32_t) * nt, hipMemcpyDeviceToHost, c->stream));
     SA_CHECK_HIP(hipStreamSynchronize(c->stream));
+    const int64_t nib = c->last_nib;
     int64_t w = 0;
-    for (int64_t ci = 0; ci < nc && w < cap; ++ci)
-      for (int64_t p = seg[ci]; p < seg[ci] + cnt[ci] && w < cap; ++p, ++w) {
-        triples[3 * w + 0] = (int32_t)(ci % c->last_nib);
-        triples[3 * w + 1] = (int32_t)(ci / c->last_nib);
+    for (int64_t ci = 0; ci < nc && w < cap; ++ci) {
+      const int64_t ib = ci % nib, jb = ci / nib;
+      // a mirrored block of a symmetric product carries its partner's triples
+      const int64_t src = (c->last_sym && ib > jb) ? jb + ib * nib : ci;
+      for (int64_t p = seg[src]; p < seg[src] + cnt[src] && w < cap; ++p, ++w) {
+        triples[3 * w + 0] = (int32_t)ib;
+        triples[3 * w + 1] = (int32_t)jb;
         triples[3 * w + 2] = kb[p];
       }
+    }
   });
 }
 

@@ -996,7 +996,7 @@ static int pick_cfg(int S, int kind, const sa_bsm* C, int tile) {
 }
 
 int64_t ozaki_multiply(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, double thresh, double alpha,
-                       const sa_opts_t& o) {
+                       const sa_opts_t& o, bool sym) {
   sa_context* c = C->ctx;
   const int kind = o.slice_type;
   SA_REQUIRE(kind == SA_SLICE_FP16 || kind == SA_SLICE_BF16 || kind == SA_SLICE_INT8, "ozaki: unknown slice type %d",
@@ -1005,7 +1005,8 @@ int64_t ozaki_multiply(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, double
   const int cfg = pick_cfg(S, kind, C, -1);
   SA_REQUIRE(cfg >= 0, "ozaki: %d %s slices not supported (FP16: 3..5, BF16: 4..6, INT8: 4..6)", S,
              kind == 2 ? "INT8" : kind == 1 ? "BF16" : "FP16");
-  const ScreenResult R = screen_and_allocate(C, A, ta, B, tb, thresh, nullptr, nullptr, 0, false);
+  TableMem mem(c);
+  const ScreenResult R = screen_and_allocate(C, A, ta, B, tb, thresh, nullptr, nullptr, 0, false, mem, sym);
   if (R.ntasks == 0) return 0;
   // slice operands; share when both roles read the same vectors of one matrix
   const sa_bsm* ra = A->base ? A->base : A;
@@ -1064,6 +1065,7 @@ int64_t ozaki_multiply(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, double
                        (const OzProd*)bprods.as<OzProd>(), items, nitems, alpha);
   }
   SA_CHECK_HIP(hipGetLastError());
+  mirror_lower(C, R);
   return R.ntasks;
 }
 

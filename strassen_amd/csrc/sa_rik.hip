@@ -26,13 +26,8 @@ namespace sa {
 void device_dgemm(sa_context* c, bool ta, bool tb, int64_t m, int64_t n, int64_t k, double alpha, const double* A,
                   int64_t lda, const double* B, int64_t ldb, double beta, double* Cm, int64_t ldc);
 void bsm_from_dense_device(sa_bsm* m, const double* d_dense, int64_t ld);
-int64_t screened_multiply(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, double thresh, double alpha,
-                          int tile_cfg, const uint8_t* d_tree, const int64_t* d_lvl_off, int nlev,
-                          bool c_must_exist);
 void check_mm_geometry(const sa_bsm* C, const sa_bsm* A, bool ta, const sa_bsm* B, bool tb);
 void beta_prepass(sa_bsm* C, double beta);
-int64_t ozaki_multiply(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, double thresh, double alpha,
-                       const sa_opts_t& o);
 
 __global__ void k_scale_copy(const double* __restrict__ src, double* __restrict__ dst, int64_t n, double s) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -366,10 +361,12 @@ int sa_ri_k_step(sa_bsm_t X, sa_bsm_t B, sa_bsm_t Bp, sa_bsm_t K, int32_t n_mu, 
       check_mm_geometry(K, view, true, view, false);
       ProfScope ps(B->ctx, "rik_k");
       beta_prepass(K, 0.0);
+      // K = B'^T B' is symmetric: the blocks above the diagonal are computed
+      // and the ones below written as their transposes
       if (ok.mode == SA_MODE_OZAKI)
-        ozaki_multiply(K, view, true, view, false, thresh, 1.0, ok);
+        ozaki_multiply(K, view, true, view, false, thresh, 1.0, ok, true);
       else
-        screened_multiply(K, view, true, view, false, thresh, 1.0, ok.tile, nullptr, nullptr, 0, false);
+        screened_multiply(K, view, true, view, false, thresh, 1.0, ok.tile, nullptr, nullptr, 0, false, true);
     } catch (...) {
       sa_bsm_destroy(view);
       throw;

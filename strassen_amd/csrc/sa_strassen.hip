@@ -31,9 +31,6 @@
 
 namespace sa {
 
-int64_t screened_multiply(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, double thresh, double alpha,
-                          int tile_cfg, const uint8_t* d_tree, const int64_t* d_lvl_off, int nlev,
-                          bool c_must_exist);
 
 static const double kStrassenFac[20] = {10, 10, 10, 25, 35, 35, 35, 35, 35, 35,
                                         35, 35, 35, 35, 35, 35, 35, 35, 35, 35};

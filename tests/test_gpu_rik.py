@@ -86,6 +86,8 @@ def test_ri_k_step_vs_oracle(mode):
     assert rel_fro(dBp.to_dense(), Bp_ref) < tol
     K = dK.to_dense()
     assert rel_fro(K, K_ref) < tol
+    # the blocks below the diagonal are the transposes of those above
+    assert np.array_equal(K, K.T)
     # K is symmetric and equals the dense contraction of the transformed tensor
     Bp = Bp_ref
     n_ic = len(w.i_chunks)
@@ -144,6 +146,7 @@ def test_ri_k_step_split_forms_vs_oracle(kind, s_t, s_k, shared_t, shared_k, win
     K = dK.to_dense()
     err = rel_fro(K, K_ref)
     assert err < 1e-10, err
+    assert np.array_equal(K, K.T)
     # per block row of K, as the split-precision error is reported
     mo = np.concatenate([[0], np.cumsum(w.mu_sizes)])
     for r in range(len(w.mu_sizes)):
