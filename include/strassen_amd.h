@@ -169,6 +169,29 @@ int sa_bsm_dot(sa_bsm_t a, sa_bsm_t b, double thresh, double* out);
  * k-step, FP64 recombination). */
 int sa_matmult(sa_bsm_t C, sa_bsm_t A, int transA, sa_bsm_t B, int transB, double thresh,
                double alpha, double beta, const sa_opts_t* opts);
+/* Allocation-free matmult (FP64).  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same semantics,
+ * screening, task list and result as sa_matmult in FP64 mode with an
+ * autotuned-or-heuristic tile and a single GEMM launch; it calls no hipMalloc,
+ * hipFree or allocating library routine and records no profiling events.
+ * Its tables (task counts and lists, GEMM records, work items, scan partials)
+ * live in one static per-device scratch region of sa_matmult_scratch_bytes()
+ * (64 MiB; held by every device the library runs on until the process ends);
+ * a multiply whose tables exceed it fails with SA_ERR_INVALID before its
+ * GEMM runs.  Output blocks that C gains must fit C's arena: reserve it
+ * beforehand with sa_bsm_reserve (else SA_ERR_INVALID).  Every call on one
+ * device shares that device's region, so calls that overlap on one device
+ * (several host threads or streams) race; the region needs no clearing
+ * between calls.  sa_tasklist_get reads the last call's list until the next
+ * multiply on the device. */
+int sa_matmult_noalloc(sa_bsm_t C, sa_bsm_t A, int transA, sa_bsm_t B, int transB, double thresh, double alpha,
+                       double beta);
+int sa_matmult_scratch_bytes(int64_t* bytes);
+/* Extension: grow a matrix's block arena to hold `elements` doubles (contents
+ * kept); lets a later sa_matmult_noalloc create blocks without allocating. */
+int sa_bsm_reserve(sa_bsm_t m, int64_t elements);
+
 /* Screened task list of the context's last sa_matmult / hybrid leaf set:
  * (ib, jb, kb) int32 triples in canonical order (C blocks column-major, kb
  * ascending).  *n receives the full count (the reference's nsig,

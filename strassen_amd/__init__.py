@@ -90,6 +90,9 @@ ABI = {
     "sa_bsm_zero": (INT, [P]),
     "sa_bsm_scale": (INT, [P, D]),
     "sa_bsm_scale_ex": (INT, [P, D, INT]),
+    "sa_bsm_reserve": (INT, [P, I64]),
+    "sa_matmult_noalloc": (INT, [P, P, INT, P, INT, D, D, D]),
+    "sa_matmult_scratch_bytes": (INT, [P]),
     "sa_bsm_calc_norms": (INT, [P]),
     "sa_bsm_recompress": (INT, [P]),
     "sa_bsm_add": (INT, [P, P, D]),

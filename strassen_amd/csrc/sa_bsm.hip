@@ -738,6 +738,17 @@ int sa_bsm_scale(sa_bsm_t m, double s) {
   });
 }
 
+int sa_bsm_reserve(sa_bsm_t m, int64_t elements) {
+  return guard([&] {
+    SA_REQUIRE(m, "null matrix");
+    SA_REQUIRE_WRITABLE(m);
+    SA_REQUIRE(elements >= 0, "reserve: negative size");
+    SA_CHECK_HIP(hipSetDevice(m->ctx->device));
+    bsm_ensure_capacity(m, elements);
+    SA_CHECK_HIP(hipStreamSynchronize(m->ctx->stream));
+  });
+}
+
 int sa_bsm_scale_ex(sa_bsm_t m, double s, int32_t norm_rule) {
   return guard([&] {
     SA_REQUIRE(m, "null matrix");

@@ -369,6 +369,7 @@ int sa_ctx_create(int device, sa_ctx_t* out) {
     SA_CHECK_HIP(hipGetDeviceProperties(&prop, device));
     c->num_cus = prop.multiProcessorCount;
     for (auto& e : c->marks) SA_CHECK_HIP(hipEventCreate(&e));
+    noalloc_setup(device);
     *out = c;
   });
 }

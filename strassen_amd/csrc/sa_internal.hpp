@@ -350,6 +350,9 @@ struct KWindows {
 // nwin == 1 (and outs == full records) when windows do not pay.
 KWindows plan_kwindows(sa_context* c, sa_bsm* A, bool ta, sa_bsm* B, bool tb, const ScreenResult& R,
                        const GemmOut* outs);
+// per-device address of the allocation-free multiply's scratch region,
+// looked up once with the device current (context creation)
+void noalloc_setup(int device);
 // learned path selector network (sa_mlp.hip)
 constexpr int kMlpFeatures = 10, kMlpOutputs = 2;
 constexpr int kMlpParams = 32 * kMlpFeatures + 32 + 32 * 32 + 32 + kMlpOutputs * 32 + kMlpOutputs;

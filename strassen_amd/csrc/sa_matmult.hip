@@ -599,6 +599,34 @@ KWindows plan_kwindows(sa_context* c, sa_bsm* A, bool ta, sa_bsm* B, bool tb, co
   return W;
 }
 
+// ------------------------------------------------ allocation-free multiply --
+// The tables of sa_matmult_noalloc live in this fixed device region.  Host
+// code never passes `scratch` itself (its host-side name is only a stand-in
+// for the device symbol): noalloc_setup looks up each device's own copy once,
+// with that device current, and the entry point uses that address.
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+static std::mutex g_scratch_mu;
+static std::vector<void*> g_scratch_ptr;  // per device
+
+void noalloc_setup(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize((size_t)device + 1, nullptr);
+  if (g_scratch_ptr[(size_t)device]) return;
+  void* p = nullptr;
+  SA_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  SA_REQUIRE(p != nullptr, "no device address for the scratch region on device %d", device);
+  g_scratch_ptr[(size_t)device] = p;
+}
+
+static void* scratch_region(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  SA_REQUIRE((int)g_scratch_ptr.size() > device && g_scratch_ptr[(size_t)device],
+             "allocation-free multiply: scratch region of device %d not set up", device);
+  return g_scratch_ptr[(size_t)device];
+}
+
 // blocksparsematrix.hpp:299-305
 void beta_prepass(sa_bsm* C, double beta) {
   if (beta == 1.0) return;
@@ -629,6 +657,34 @@ int sa_matmult(sa_bsm_t C, sa_bsm_t A, int transA, sa_bsm_t B, int transB, doubl
     } else {
       SA_REQUIRE(false, "matmult: unknown mode %d", o.mode);
     }
+  });
+}
+
+int sa_matmult_noalloc(sa_bsm_t C, sa_bsm_t A, int transA, sa_bsm_t B, int transB, double thresh, double alpha,
+                       double beta) {
+  return guard([&] {
+    SA_REQUIRE(C && A && B, "null matrix");
+    check_mm_geometry(C, A, transA != 0, B, transB != 0);
+    sa_context* c = C->ctx;
+    SA_CHECK_HIP(hipSetDevice(c->device));
+    TableMem mem(c, scratch_region(c->device), kScratchBytes);
+    c->quiet = true;  // no profiling events
+    try {
+      beta_prepass(C, beta);
+      screened_multiply_mem(C, A, transA != 0, B, transB != 0, thresh, alpha, -1, nullptr, nullptr, 0, false, false,
+                            mem);
+    } catch (...) {
+      c->quiet = false;
+      throw;
+    }
+    c->quiet = false;
+  });
+}
+
+int sa_matmult_scratch_bytes(int64_t* bytes) {
+  return guard([&] {
+    SA_REQUIRE(bytes, "null argument");
+    *bytes = (int64_t)kScratchBytes;
   });
 }
 
