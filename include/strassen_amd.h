@@ -119,6 +119,11 @@ int sa_bsm_copy(sa_bsm_t dst, sa_bsm_t src);
 int sa_bsm_from_dense(sa_bsm_t m, const double* host, int64_t ld);
 /* to_pointer (blocksparsematrix.hpp:103-124): writes present blocks only. */
 int sa_bsm_to_dense(sa_bsm_t m, double* host, int64_t ld);
+/* Dense values of the block sub-grid rows [rb0, rb1) x columns [cb0, cb1)
+ * (absent blocks as zeros) into host (column-major, leading dimension ld >=
+ * the sub-grid's rows): the reference's per-block read access block(rb, cb)
+ * (blocksparsematrix.h:71-72) for one block or a range. */
+int sa_bsm_get_blocks(sa_bsm_t m, int32_t rb0, int32_t rb1, int32_t cb0, int32_t cb1, double* host, int64_t ld);
 /* Block presence (1/0) and cached norms, grid order. Either may be NULL. */
 int sa_bsm_get_meta(sa_bsm_t m, uint8_t* present, double* norms);
 /* Extension: synthetic input generated in HBM.  Block b gets

@@ -91,6 +91,7 @@ ABI = {
     "sa_bsm_scale": (INT, [P, D]),
     "sa_bsm_scale_ex": (INT, [P, D, INT]),
     "sa_bsm_reserve": (INT, [P, I64]),
+    "sa_bsm_get_blocks": (INT, [P, I32, I32, I32, I32, P, I64]),
     "sa_matmult_noalloc": (INT, [P, P, INT, P, INT, D, D, D]),
     "sa_matmult_scratch_bytes": (INT, [P]),
     "sa_bsm_calc_norms": (INT, [P]),
@@ -242,6 +243,11 @@ class BSM:
                                                C.byref(self.h)))
         _live_bsm.add(self)
         self.nrow, self.ncol, self.nrb, self.ncb, _, _ = self.info(False)
+        if _uniform is not None:  # target block size, ragged last block
+            self.rsz = np.array([min(bsr, nrow - i * bsr) for i in range(self.nrb)], dtype=np.int64)
+            self.csz = np.array([min(bsc, ncol - j * bsc) for j in range(self.ncb)], dtype=np.int64)
+        else:
+            self.rsz, self.csz = np.asarray(rsz, dtype=np.int64), np.asarray(csz, dtype=np.int64)
 
     @classmethod
     def uniform(cls, ctx, nrow, ncol, bs_r, bs_c, thresh=0.0):
@@ -281,6 +287,16 @@ class BSM:
     def to_dense(self):
         out = np.zeros((self.nrow, self.ncol), dtype=np.float64, order="F")
         _check(lib().sa_bsm_to_dense(self.h, _ptr(out), self.nrow))
+        return out
+
+    def blocks(self, rb0, rb1, cb0, cb1):
+        """Dense values of block rows [rb0, rb1) x block columns [cb0, cb1)
+        (absent blocks zero); the reference's block(rb, cb) for one block."""
+        rows = int(np.sum(self.rsz[rb0:rb1]))
+        cols = int(np.sum(self.csz[cb0:cb1]))
+        out = np.zeros((rows, cols), order="F")
+        if rows and cols:
+            _check(lib().sa_bsm_get_blocks(self.h, rb0, rb1, cb0, cb1, _ptr(out), rows))
         return out
 
     def meta(self):

@@ -3,12 +3,13 @@
 //
 // Same constructors, members and free functions (matmult, dot); every body
 // calls the C ABI in include/strassen_amd.h.  Blocks are not host objects, so
-// the reference's per-block accessors (blocks(), block(rb, cb)) are replaced by
-// block_present() / block_norm(); dense access goes through to_matrix() /
-// to_pointer().  The memory pool argument is accepted for source compatibility:
+// block(rb, cb) returns a host copy (read access) and blocks() is not offered;
+// block_present() / block_norm() read the metadata alone, and dense access
+// goes through to_matrix() / to_pointer().  The memory pool argument is accepted for source compatibility:
 // device storage is a per-matrix arena (see blockallocator.h).
 #ifndef SA_BLOCKSPARSEMATRIX_H
 #define SA_BLOCKSPARSEMATRIX_H
+#include <algorithm>
 #include <memory>
 #include <vector>
 
@@ -152,6 +153,20 @@ class BlockSparseMatrix final {
     sa_cpp::check(sa_bsm_frobenius_norm(_h, &r), "sa_bsm_frobenius_norm");
     return r;
   }
+  // blocksparsematrix.h:71-72, read access: a host copy of block (rb, cb),
+  // 0 x 0 when the block is absent, with the block's cached norm
+  Matrix<Num, std::allocator<Num>> block(size_t rb, size_t cb) const {
+    const auto m = meta();
+    const size_t b = rb + cb * _nrb;
+    if (!m.first[b]) return Matrix<Num, std::allocator<Num>>();
+    const size_t r = block_rows(rb), c = block_cols(cb);
+    Matrix<Num, std::allocator<Num>> out(r, c);
+    sa_cpp::check(sa_bsm_get_blocks(_h, (int32_t)rb, (int32_t)rb + 1, (int32_t)cb, (int32_t)cb + 1, out.data_ptr(),
+                                    (int64_t)r),
+                  "sa_bsm_get_blocks");
+    out.frobenius_norm() = m.second[b];
+    return out;
+  }
   // per-block metadata (replaces block(rb, cb).size() != 0 / .frobenius_norm())
   bool block_present(size_t rb, size_t cb) const { return meta().first[rb + cb * _nrb] != 0; }
   Num block_norm(size_t rb, size_t cb) const { return meta().second[rb + cb * _nrb]; }
@@ -172,6 +187,14 @@ class BlockSparseMatrix final {
   std::shared_ptr<BlockMemoryPool<Num>> _pool;
   std::vector<int32_t> _rsz, _csz;  // uneven geometry (empty for uniform)
 
+  size_t block_rows(size_t rb) const {
+    if (!_rsz.empty()) return (size_t)_rsz[rb];
+    return std::min(_bs_r, _nrow - rb * _bs_r);
+  }
+  size_t block_cols(size_t cb) const {
+    if (!_csz.empty()) return (size_t)_csz[cb];
+    return std::min(_bs_c, _ncol - cb * _bs_c);
+  }
   void refresh_shape() {
     int64_t nr = 0, nc = 0;
     int32_t rb = 0, cb = 0;

@@ -134,6 +134,22 @@ __global__ void __launch_bounds__(256) k_arena_to_dense(const double* __restrict
   for (int64_t e = threadIdx.x; e < n; e += blockDim.x) dst[(e % nr) + (e / nr) * ld] = src[e];
 }
 
+// blocks of rows [rb0, rb0 + nrb) x columns [cb0, cb0 + ncb) into a dense
+// sub-matrix whose (0, 0) is the first element of block (rb0, cb0)
+__global__ void __launch_bounds__(256) k_arena_to_dense_sub(const double* __restrict__ arena,
+                                                            const int64_t* __restrict__ off, Geo g, int32_t rb0,
+                                                            int32_t nrb, int32_t cb0, double* __restrict__ dense,
+                                                            int64_t ld) {
+  const int rb = rb0 + (int)(blockIdx.x % nrb), cb = cb0 + (int)(blockIdx.x / nrb);
+  const int64_t o = off[rb + (int64_t)cb * g.nrb];
+  if (o < 0) return;
+  const int nr = g.rsz[rb], nc = g.csz[cb];
+  const int64_t n = (int64_t)nr * nc;
+  double* dst = dense + (g.roff[rb] - g.roff[rb0]) + (g.coff[cb] - g.coff[cb0]) * ld;
+  const double* src = arena + o;
+  for (int64_t e = threadIdx.x; e < n; e += blockDim.x) dst[(e % nr) + (e / nr) * ld] = src[e];
+}
+
 // --- synthetic generator (definition shared with strassen_amd/synth.py) ----
 __device__ inline uint64_t splitmix64(uint64_t x) {
   uint64_t z = x + 0x9E3779B97F4A7C15ull;
@@ -648,6 +664,33 @@ int sa_bsm_to_dense(sa_bsm_t m, double* host, int64_t ld) {
     hipLaunchKernelGGL(k_arena_to_dense, dim3((unsigned)m->nb()), dim3(256), 0, c->stream, (const double*)m->data(),
                        (const int64_t*)m->d_off, geo(m), c->ws_stage.as<double>(), ld);
     SA_CHECK_HIP(hipMemcpyAsync(host, c->ws_stage.p, bytes, hipMemcpyDeviceToHost, c->stream));
+    SA_CHECK_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int sa_bsm_get_blocks(sa_bsm_t m, int32_t rb0, int32_t rb1, int32_t cb0, int32_t cb1, double* host, int64_t ld) {
+  return guard([&] {
+    SA_REQUIRE(m, "null matrix");
+    SA_REQUIRE(0 <= rb0 && rb0 <= rb1 && rb1 <= m->nrb && 0 <= cb0 && cb0 <= cb1 && cb1 <= m->ncb,
+               "get_blocks: block range [%d, %d) x [%d, %d) outside the %d x %d grid", rb0, rb1, cb0, cb1, m->nrb,
+               m->ncb);
+    if (rb0 == rb1 || cb0 == cb1) return;
+    const int64_t rows = m->roff[(size_t)rb1 - 1] + m->rsz[(size_t)rb1 - 1] - m->roff[(size_t)rb0];
+    const int64_t cols = m->coff[(size_t)cb1 - 1] + m->csz[(size_t)cb1 - 1] - m->coff[(size_t)cb0];
+    SA_REQUIRE(host && ld >= rows, "get_blocks: null output or ld < %lld rows", (long long)rows);
+    sa_context* c = m->ctx;
+    SA_CHECK_HIP(hipSetDevice(c->device));
+    const int64_t elems = ld * (cols - 1) + rows;
+    c->ws_stage.reserve(sizeof(double) * (size_t)elems);
+    fill_f64(c, c->ws_stage.as<double>(), elems, 0.0);  // absent blocks read as zero
+    const int32_t nr = rb1 - rb0, nc = cb1 - cb0;
+    hipLaunchKernelGGL(k_arena_to_dense_sub, dim3((unsigned)((int64_t)nr * nc)), dim3(256), 0, c->stream,
+                       (const double*)m->data(), (const int64_t*)m->d_off, geo(m), rb0, nr, cb0,
+                       c->ws_stage.as<double>(), ld);
+    // column by column: rows beyond the sub-matrix in each column of the
+    // caller's array are left alone
+    SA_CHECK_HIP(hipMemcpy2DAsync(host, sizeof(double) * (size_t)ld, c->ws_stage.p, sizeof(double) * (size_t)ld,
+                                  sizeof(double) * (size_t)rows, (size_t)cols, hipMemcpyDeviceToHost, c->stream));
     SA_CHECK_HIP(hipStreamSynchronize(c->stream));
   });
 }

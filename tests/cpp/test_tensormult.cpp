@@ -9,6 +9,7 @@
 //   form2_<l>.data  C^l = alpha op(A^0) op(B^l) + beta C0             (transC = false)
 //   form2t_<l>.data (C^l)^T = alpha op(A^0) op(B^l) + beta C0^T, C^l written as stored
 //   form4_<l>.data  the reference's fourth overload on C^l = C0
+//   block_last.data block(last, last) of form 1's C (empty file if absent)
 #include <cstdio>
 #include <string>
 #include <vector>
@@ -55,6 +56,10 @@ int main(int argc, char** argv) {
       BlockSparseMatrix<double> C = fresh();
       tensormult(C, A, tA, B, tB, thresh, alpha, beta);
       save(C, "form1.data");
+      // per-block read access: the last block row / column (ragged)
+      const size_t rb = C.nrowblocks() - 1, cb = C.ncolblocks() - 1;
+      const auto blk = C.block(rb, cb);
+      blk.write_to_file(path("block_last.data").c_str());
     }
     {
       BlockSparse3DTensor<double> C = fresh_tensor();

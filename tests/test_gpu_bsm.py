@@ -162,3 +162,17 @@ def test_scale_with_recomputed_norms():
         d2 = sa.BSM.uniform(ctx(), a.shape[0], a.shape[1], 7, 9, 0.0).from_dense(a)
         d2.scale(s)
         assert_same_meta(o, d2)  # the reference rule, bit-exact with the oracle
+
+
+def test_get_blocks_sub_grid():
+    """sa_bsm_get_blocks (block(rb, cb) read access) against the dense copy, on
+    uneven blocks with absent ones, single blocks and ranges."""
+    rng = np.random.default_rng(31)
+    rsz, csz = [5, 9, 14, 3], [7, 2, 11]
+    a = rand_dense(rng, rsz, csz, 0.5)
+    o, d = pair_blocked(a, rsz, csz, 0.0)
+    full = d.to_dense()
+    ro, co = np.concatenate([[0], np.cumsum(rsz)]), np.concatenate([[0], np.cumsum(csz)])
+    for rb0, rb1, cb0, cb1 in [(0, 4, 0, 3), (1, 2, 2, 3), (3, 4, 0, 1), (1, 3, 1, 3), (2, 2, 0, 3)]:
+        got = d.blocks(rb0, rb1, cb0, cb1)
+        assert np.array_equal(got, full[ro[rb0]:ro[rb1], co[cb0]:co[cb1]])

@@ -50,7 +50,16 @@ def test_tensormult_forms_vs_oracle(tmp_path, ta, tb, alpha, beta):
         oB = oracle.OBsm.uniform(n, n, bs, bs, thresh).from_dense(B[l])
         oracle.matmult(oC, oA, ta, oB, tb, thresh, alpha, beta if l == 0 else 1.0)
     want1 = oC.to_dense()
-    assert rel_fro(read("form1.data"), want1) < 1e-12
+    got1 = read("form1.data")
+    assert rel_fro(got1, want1) < 1e-12
+    # block(rb, cb) of the drop-in: the ragged last block (n = 50 = 3 * 16 + 2)
+    last = np.fromfile(tmp_path / "block_last.data")
+    if beta != 0.0:  # C0's blocks are all present
+        assert last.size == 4
+    if last.size == 0:  # absent: screened out and no C0 block kept (beta = 0)
+        assert not np.any(got1[48:, 48:])
+    else:
+        assert last.size == 4 and np.array_equal(last.reshape(2, 2, order="F"), got1[48:, 48:])
     # the screened sum differs from the unscreened one (screening is exercised)
     full = sum((A[l].T if ta else A[l]) @ (B[l].T if tb else B[l]) for l in range(L)) * alpha + beta * c0
     assert rel_fro(want1, full) > 1e-8
