@@ -216,8 +216,10 @@ def bench_rik(args):
     o64 = sa.default_opts()
     # kind suffix "s": the transform's operands sliced with shared exponents (one
     # int32 accumulation and one FP64 fold per output instead of one per product)
+    # suffix "ss": shared exponents in the K contraction as well
     kinds = {"fp16": sa.SLICE_FP16, "bf16": sa.SLICE_BF16, "int8": sa.SLICE_INT8,
-             "fp16s": sa.SLICE_FP16, "bf16s": sa.SLICE_BF16, "int8s": sa.SLICE_INT8}
+             "fp16s": sa.SLICE_FP16, "bf16s": sa.SLICE_BF16, "int8s": sa.SLICE_INT8,
+             "int8ss": sa.SLICE_INT8}
     forms = []
     for f in args.forms.split(","):
         parts = f.split(":")
@@ -275,7 +277,8 @@ def bench_rik(args):
         name = f"{kind}_{s_t}_{s_k}" + ("" if tiles == (-1, -1) else f"_t{tiles[0]}_{tiles[1]}")
         oz_t = sa.default_opts(mode=sa.MODE_OZAKI, slices=s_t, slice_type=kinds[kind], tile=tiles[0],
                                shared_exp=1 if kind.endswith("s") else 0)
-        oz_k = sa.default_opts(mode=sa.MODE_OZAKI, slices=s_k, slice_type=kinds[kind], tile=tiles[1])
+        oz_k = sa.default_opts(mode=sa.MODE_OZAKI, slices=s_k, slice_type=kinds[kind], tile=tiles[1],
+                               shared_exp=1 if kind.endswith("ss") else 0)
 
         def stepmix():
             sa.ri_k_step(dX, dB, dBp, dK, w.mu_sizes, w.i_chunks, w.thresh, oz_t, oz_k)
