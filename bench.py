@@ -61,6 +61,25 @@ def tasks_flops(t, rsz, ksz, csz):
     return float((2.0 * r[t[:, 0]] * c[t[:, 1]] * k[t[:, 2]]).sum())
 
 
+def tasks_issued_flops(t, rsz, ksz, csz, tm, tn, kq=4):
+    """MFMA flops the FP64 kernels issue for the task list: each output block
+    is covered by whole tm x tn tiles and each product's k is padded to the
+    MFMA k (4)."""
+    r, k, c = (np.asarray(x, dtype=np.float64) for x in (rsz, ksz, csz))
+    rp = np.ceil(r / tm) * tm
+    cp = np.ceil(c / tn) * tn
+    kp = np.ceil(k / kq) * kq
+    return float((2.0 * rp[t[:, 0]] * cp[t[:, 1]] * kp[t[:, 2]]).sum())
+
+
+def tile_dims(name):
+    """(tm, tn) of an FP64 tile config name such as 'stage16x32_k16_n4'."""
+    import re
+
+    mt = re.search(r"(\d+)x(\d+)", name or "")
+    return (int(mt.group(1)), int(mt.group(2))) if mt else (16, 16)
+
+
 def timed(ctx, fn, steps):
     ctx.profile_reset()
     ctx.mark(0)
@@ -297,7 +316,9 @@ def bench_rik(args):
         pairs = s_t * (s_t + 1) // 2 * f_tr + s_k * (s_k + 1) // 2 * f_k_exec
         peak = PEAK_INT8 if kind.startswith("int8") else PEAK_FP16
         ach = pairs / (goz_ms / args.steps * 1e-3) / 1e12
-        results[name] = {"slice_type": kind, "slices_transform": s_t, "slices_k": s_k, "step_ms": round(ms, 3),
+        results[name] = {"slice_type": kind.rstrip("s"), "slices_transform": s_t, "slices_k": s_k,
+                         "shared_exponents": {"transform": kind.endswith("s"), "k": kind.endswith("ss")},
+                         "step_ms": round(ms, 3),
                          "speedup_vs_fp64_gpu": round(ms64 / ms, 3), "rel_err_K_vs_fp64": err,
                          "rel_err_K_per_block_row": {"max": max(rows_err), "median": float(np.median(rows_err))},
                          "meets_1e-10": bool(err <= 1e-10), "ms_per_step": prof,
@@ -611,11 +632,13 @@ def bench_gemm(args):
     cfg_name = ctx.gemm_config()[1] if not ozaki else f"ozaki_fp16_s{args.slices_k}"
     dense = 2.0 * n ** 3
     g_avg = g_ms / max(g_n, 1)
+    issued = None
     if ozaki:
         pairs = args.slices_k * (args.slices_k + 1) // 2
         achieved, peak = pairs * flops_alg / (g_avg * 1e-3) / 1e12, PEAK_FP16
     else:
         achieved, peak = flops_alg / (g_avg * 1e-3) / 1e12, PEAK_FP64
+        issued = tasks_issued_flops(t, w.rsz, w.ksz, w.csz, *tile_dims(cfg_name))
     rec = {
         "metric": METRIC, "value": round(dense / (ms * 1e-3) / 1e12, 3), "unit": "dense-equivalent TFLOP/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
@@ -625,7 +648,9 @@ def bench_gemm(args):
         "config": {"workload": w.name, "n": n, "nblocks": int(len(w.rsz)), "thresh": w.thresh, "tasks": int(len(t)),
                    "algorithmic_flop": flops_alg, "gemm_tile": cfg_name,
                    "gemm_tile_choice": "forced" if args.tile >= 0 else "autotuned", "parallelism": "single GPU",
-                   "rel_err_vs_fp64": err},
+                   "rel_err_vs_fp64": err, "issued_mfma_flop": issued,
+                   "useful_frac_of_issued": round(flops_alg / issued, 4) if issued else None,
+                   "issued_tflops": round(issued / (g_avg * 1e-3) / 1e12, 3) if issued else None},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
                      "frac": round(achieved / peak, 4), "traffic": None,
                      "kernel": "k_gemm_ozaki" if ozaki else "k_gemm_fp64", "kernel_avg_ms": round(g_avg, 4),
@@ -665,10 +690,10 @@ def main():
     ap.add_argument("--workload", default="rik", choices=["rik", "band", "atoms", "band_split", "hybrid", "occ"])
     ap.add_argument("--tile", type=int, default=-1)
     ap.add_argument("--slices-k", type=int, default=4, help="split-precision slices (band_split workload)")
-    ap.add_argument("--forms", default="int8:5:5,fp16:4:4",
+    ap.add_argument("--forms", default="int8ss:5:5,int8:5:5,fp16:4:4",
                     help="RI-K split-precision forms kind:slices_transform:slices_k[:tile_transform:tile_k] "
-                         "(kind fp16|bf16|int8, suffix s for shared exponents in the transform, e.g. int8s; "
-                         "tiles = split-precision kernel table ids, default chosen)")
+                         "(kind fp16|bf16|int8, suffix s for shared exponents in the transform, ss in both "
+                         "stages, e.g. int8ss; tiles = split-precision kernel table ids, default chosen)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--teams", type=int, default=8, help="occ: team members (streams, host threads)")
     ap.add_argument("--n-occ", type=int, default=200, help="RI-K occupied orbitals (200 = the headline workload)")

@@ -116,6 +116,112 @@ __global__ void __launch_bounds__(64 * WM * WN)
     }
 }
 
+// Small (atom-shaped, 5-40 wide) blocks: each wave owns a (16 FM) x (16 FN)
+// output tile on its own (no LDS, no barriers) and walks the stream of stages
+// of its output -- KC 4-k steps of one product, the whole product when k <= 4
+// KC -- with operands loaded straight from global memory into registers and
+// the next stage's loads issued before the current stage's MFMAs.  A stage
+// then costs one memory latency instead of one per 4-k step, and with few
+// registers per wave (KC = 4: about 60) eight waves per SIMD cover it.
+// 4-k steps past the product's k are skipped (wave-uniform branch).
+template <int FM, int FN, int KC, int NW, bool TA, bool TB>
+__global__ void __launch_bounds__(64 * NW)
+    k_gemm_fp64_stage(const GemmOut* __restrict__ outs, const GemmProd* __restrict__ prods,
+                      const GemmItem* __restrict__ items, int64_t nitems, double alpha) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t nblk = (nitems + NW - 1) / NW;
+  const int64_t it = xcd_remap(blockIdx.x, nblk) * NW + w;
+  if (it >= nitems) return;
+  const GemmItem item = items ? items[it] : GemmItem{(int32_t)it, 0, 0};
+  const GemmOut o = outs[item.out];
+  const int m = o.m, n = o.n;
+  const int m0 = item.tm * 16 * FM, n0 = item.tn * 16 * FN;
+  const int li = lane & 15, lk = lane >> 4;
+
+  dbl4 acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
+
+  // operands of 4-k steps s < KC of the stage at k0 of product pr: lane l
+  // holds op(A)[row l % 16][k0 + 4 s + l / 16] and op(B)[k0 + 4 s + l / 16][col l % 16]
+  auto load = [&](const GemmProd& pr, int k0, double (&a)[KC][FM], double (&b)[KC][FN]) {
+    const int k = pr.k;
+#pragma unroll
+    for (int s = 0; s < KC; ++s) {
+      const int kk = k0 + 4 * s + lk;
+#pragma unroll
+      for (int i = 0; i < FM; ++i) {
+        const int r = m0 + i * 16 + li;
+        a[s][i] = (r < m && kk < k) ? (TA ? pr.a[kk + (int64_t)r * k] : pr.a[r + (int64_t)kk * m]) : 0.0;
+      }
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        const int cc = n0 + j * 16 + li;
+        b[s][j] = (cc < n && kk < k) ? (TB ? pr.b[cc + (int64_t)kk * n] : pr.b[kk + (int64_t)cc * k]) : 0.0;
+      }
+    }
+  };
+
+  int p = o.p0, k0 = 0;
+  if (p < o.p1) {
+    GemmProd pr = prods[p];
+    double ac[KC][FM], bc[KC][FN], an[KC][FM], bn[KC][FN];
+    load(pr, 0, ac, bc);
+    while (true) {
+      int np = p, nk0 = k0 + 4 * KC;
+      GemmProd pn = pr;
+      if (nk0 >= pr.k) {
+        np = p + 1;
+        nk0 = 0;
+        if (np < o.p1) pn = prods[np];
+      }
+      if (np < o.p1) load(pn, nk0, an, bn);
+      const int ns = (int)ceil_div(pr.k - k0, 4);  // live 4-k steps of this stage (>= 1)
+#pragma unroll
+      for (int s = 0; s < KC; ++s) {
+        if (s < ns) {
+#pragma unroll
+          for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+              acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ac[s][i], bc[s][j], acc[i][j], 0, 0, 0);
+        }
+      }
+      if (np >= o.p1) break;
+#pragma unroll
+      for (int s = 0; s < KC; ++s) {
+#pragma unroll
+        for (int i = 0; i < FM; ++i) ac[s][i] = an[s][i];
+#pragma unroll
+        for (int j = 0; j < FN; ++j) bc[s][j] = bn[s][j];
+      }
+      p = np;
+      k0 = nk0;
+      pr = pn;
+    }
+  }
+
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const int col = n0 + j * 16 + li;
+      if (col >= n) continue;
+      double* cp = o.c + (int64_t)col * m;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = m0 + i * 16 + 4 * v + lk;
+        if (row < m) {
+          double x = alpha * acc[i][j][v];
+          if (o.acc) x = x + cp[row];
+          cp[row] = x;
+        }
+      }
+    }
+}
+
 // Pipelined variant for large blocks: the workgroup walks the flattened stream
 // of (product, k-slice) stages of its output tile; while the MFMAs of stage s
 // read LDS buffer s%2, the global loads of stage s+1 are already in flight into
@@ -400,6 +506,13 @@ struct Cfg {
         NW                                                                                          \
   }
 
+#define SA_SCFG(NAME, FM, FN, KC, NW)                                                                       \
+  Cfg {                                                                                                   \
+    NAME, 16 * FM, 16 * FN, 64 * NW,                                                                      \
+        {k_gemm_fp64_stage<FM, FN, KC, NW, false, false>, k_gemm_fp64_stage<FM, FN, KC, NW, false, true>, \
+         k_gemm_fp64_stage<FM, FN, KC, NW, true, false>, k_gemm_fp64_stage<FM, FN, KC, NW, true, true>},  \
+        NW                                                                                                \
+  }
 #define SA_QCFG(NAME, WM, WN, FM, FN, KT)                                                                 \
   Cfg {                                                                                                   \
     NAME, 16 * FM * WM, 16 * FN * WN, 64 * WM * WN,                                                       \
@@ -438,6 +551,12 @@ static const Cfg kCfgs[] = {
     SA_QCFG("q4pipe128x64_k16", 2, 2, 4, 2, 16),   // 19
     SA_QCFG("q4pipe64x64_k16", 2, 2, 2, 2, 16),    // 20
     SA_QCFG("q4pipe64x128_k16", 2, 2, 2, 4, 16),   // 21
+    SA_SCFG("stage16x16_k16_n4", 1, 1, 4, 4),      // 22: wave per tile, whole-stage register loads
+    SA_SCFG("stage16x16_k32_n4", 1, 1, 8, 4),      // 23
+    SA_SCFG("stage32x32_k16_n4", 2, 2, 4, 4),      // 24
+    SA_SCFG("stage16x32_k16_n4", 1, 2, 4, 4),      // 25
+    SA_SCFG("stage32x16_k16_n4", 2, 1, 4, 4),      // 26
+    SA_SCFG("stage48x48_k8_n4", 3, 3, 2, 4),       // 27
 };
 static constexpr int kNumCfgs = (int)(sizeof(kCfgs) / sizeof(kCfgs[0]));
 

@@ -53,7 +53,7 @@ def test_alpha_beta_existing_c(beta):
     check_mm(oC, dC, oA, dA, 0, oB, dB, 0, 1e-4, -1.25, beta)
 
 
-@pytest.mark.parametrize("tile", list(range(22)))
+@pytest.mark.parametrize("tile", list(range(28)))
 def test_every_tile_config(tile):
     rng = np.random.default_rng(11)
     w = synth.small_uneven(n=500, fill=0.3, seed=4)
@@ -66,7 +66,7 @@ def test_every_tile_config(tile):
     assert n > 100
 
 
-@pytest.mark.parametrize("tile", [14, 15, 16, 17, 18, 19, 20, 21])
+@pytest.mark.parametrize("tile", [14, 15, 16, 17, 18, 19, 20, 21, 22, 23, 24, 25, 26, 27])
 @pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
 def test_pipelined_tiles_transposes(tile, ta, tb):
     """Large ragged blocks (150, with 7/3/149-wide edges) span several pipeline

@@ -115,6 +115,7 @@ def test_ri_k_step_vs_oracle(mode):
     ("int8", 5, 5, 0, 0, 0),   # the same in one launch
     ("int8", 5, 5, 1, 0, 0),   # shared exponents in the transform
     ("int8", 5, 5, 0, 1, 1),   # shared exponents on the K contraction's one shared slicing
+    ("int8", 5, 5, 1, 1, 1),   # shared exponents in both stages (the fastest benchmarked form)
     ("bf16", 5, 5, 0, 0, 1),
     ("fp16", 4, 4, 0, 0, 1),
 ])
