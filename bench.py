@@ -91,7 +91,9 @@ def timed(ctx, fn, steps):
 
 
 # ----------------------------------------------------------------- RI-K --
-RIK_PROFILE = "profiles/gemm_ozaki_int8_ri_k_step.md"
+# rocprofv3 summaries of the RI-K step's split-precision kernel, per form
+RIK_PROFILES = {"int8ss_5_5": "profiles/gemm_ozaki_int8_shared_ri_k_step.md",
+                "int8_5_5": "profiles/gemm_ozaki_int8_ri_k_step.md"}
 
 
 def profiled_traffic(path):
@@ -361,7 +363,7 @@ def bench_rik(args):
         "roofline": {"bound": "mfma", "achieved": H["lowp_tops"], "peak": H["lowp_peak"],
                      "unit": "TOP/s" if H["slice_type"].startswith("int8") else "TFLOP/s",
                      "frac": H["lowp_frac"], "kernel": "k_gemm_ozaki",
-                     **profiled_traffic(RIK_PROFILE if (args.n_occ == 200 and head == "int8_5_5") else None),
+                     **profiled_traffic(RIK_PROFILES.get(head) if args.n_occ == 200 else None),
                      "kernel_avg_ms": H["kernel_avg_ms"],
                      "note": "achieved = executed low-precision slice-product operations (slice pairs x algorithmic "
                              "flop of the computed products: the transform's and K's upper triangle) / k_gemm_ozaki "

@@ -122,7 +122,7 @@ __global__ void __launch_bounds__(64 * WM * WN)
 // KC -- with operands loaded straight from global memory into registers and
 // the next stage's loads issued before the current stage's MFMAs.  A stage
 // then costs one memory latency instead of one per 4-k step, and with few
-// registers per wave (KC = 4: about 60) eight waves per SIMD cover it.
+// registers per wave (KC = 2 or 4: under 72) seven or eight waves per SIMD cover it.
 // 4-k steps past the product's k are skipped (wave-uniform branch).
 template <int FM, int FN, int KC, int NW, bool TA, bool TB>
 __global__ void __launch_bounds__(64 * NW)
@@ -552,7 +552,7 @@ static const Cfg kCfgs[] = {
     SA_QCFG("q4pipe64x64_k16", 2, 2, 2, 2, 16),    // 20
     SA_QCFG("q4pipe64x128_k16", 2, 2, 2, 4, 16),   // 21
     SA_SCFG("stage16x16_k16_n4", 1, 1, 4, 4),      // 22: wave per tile, whole-stage register loads
-    SA_SCFG("stage16x16_k32_n4", 1, 1, 8, 4),      // 23
+    SA_SCFG("stage16x16_k8_n4", 1, 1, 2, 4),       // 23
     SA_SCFG("stage32x32_k16_n4", 2, 2, 4, 4),      // 24
     SA_SCFG("stage16x32_k16_n4", 1, 2, 4, 4),      // 25
     SA_SCFG("stage32x16_k16_n4", 2, 1, 4, 4),      // 26
@@ -608,7 +608,7 @@ static void launch_cfg(sa_context* c, const GemmBatch& b, int cfg) { gemm_fp64_l
 // ---------------------------------------------------------------- autotuner --
 // The tile configuration is chosen per shape class (transposes, largest block
 // dimensions) by timing every plausible configuration on a sample of the real
-// batch (at most 8192 outputs taken at a uniform stride) whose outputs are redirected to
+// batch (at most 32768 outputs taken at a uniform stride) whose outputs are redirected to
 // scratch with overwrite semantics, so the caller's C is untouched.  The
 // choice is cached for the process.
 __global__ void k_redirect_outs(const GemmOut* __restrict__ in, int64_t n, int64_t stride, double* __restrict__ scratch,
@@ -641,10 +641,12 @@ static int autotune(sa_context* c, const GemmBatch& b) {
     auto it = tune_cache().find(key);
     if (it != tune_cache().end()) return it->second;
   }
-  // sample size bounded by work: about 4e9 flops of products (at least 64 outputs)
+  // sample size bounded by work: about 1.6e10 flops of products (64 to 32768
+  // outputs) -- small-block batches need samples of a few hundred
+  // microseconds before the timings rank the configs as the full batch does
   const double per_out = 2.0 * b.max_m * b.max_n * (double)b.max_k * ((double)b.nprods / (double)b.nouts);
   const int64_t S = std::max<int64_t>(1, std::min<int64_t>(
-                        b.nouts, std::min<int64_t>(8192, std::max<int64_t>(64, (int64_t)(4e9 / std::max(per_out, 1.0))))));
+                        b.nouts, std::min<int64_t>(32768, std::max<int64_t>(64, (int64_t)(1.6e10 / std::max(per_out, 1.0))))));
   const int64_t stride = b.nouts / S;
   const int64_t slot = (int64_t)b.max_m * b.max_n;
   DevBuf& to = c->buf("tune_outs");
