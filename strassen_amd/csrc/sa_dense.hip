@@ -37,21 +37,6 @@ void device_dgemm(sa_context* c, bool ta, bool tb, int64_t m, int64_t n, int64_t
   }
 }
 
-void device_dgemm_strided_batched(sa_context* c, int64_t m, int64_t n, int64_t k, const double* A, int64_t lda,
-                                  int64_t sA, const double* B, int64_t ldb, int64_t sB, double* Cm, int64_t ldc,
-                                  int64_t sC, int64_t count) {
-  rocblas_handle h = handle_for(c);
-  const double alpha = 1.0, beta = 0.0;
-  const rocblas_status st = rocblas_dgemm_strided_batched(
-      h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)m, (rocblas_int)n, (rocblas_int)k, &alpha, A,
-      (rocblas_int)lda, (rocblas_stride)sA, B, (rocblas_int)ldb, (rocblas_stride)sB, &beta, Cm, (rocblas_int)ldc,
-      (rocblas_stride)sC, (rocblas_int)count);
-  if (st != rocblas_status_success) {
-    set_error("rocblas_dgemm_strided_batched failed (%d)", (int)st);
-    throw Error{SA_ERR_HIP};
-  }
-}
-
 __global__ void __launch_bounds__(256) k_vec_norm_dot(const double* __restrict__ x, const double* __restrict__ y,
                                                       int64_t n, double* __restrict__ out) {
   // single wave, canonical 64-lane strided order (same as the block norm)

@@ -229,9 +229,10 @@ __global__ void __launch_bounds__(64 * NW)
 // Rows of the staging buffers carry one pad element, so the k-major staging
 // writes of transposed operands hit distinct LDS banks.
 // Q4: the same tile on v_mfma_f64_4x4x4f64 (four 4 x 4 x 4 blocks per
-// instruction), which issues about 1.5x the FP64 rate of v_mfma_f64_16x16x4f64
-// on gfx950 (measured: ~73 against ~48 TFLOP/s for back-to-back independent
-// MFMAs).  Block g of an instruction takes A[i][k] from lane 16 k + 4 g + i,
+// instruction).  With the same tiles and staging it runs faster than the
+// v_mfma_f64_16x16x4f64 form here (measured on gfx950: dense 2048-blocks
+// 51-54 against 46-49 TFLOP/s with the 128 x 64 tile, the RI-K step's FP64
+// GEMMs about 6 % faster).  Block g of an instruction takes A[i][k] from lane 16 k + 4 g + i,
 // B[k][j] from lane 16 k + 4 g + j and returns D[i][j] in lane 16 i + 4 g + j.
 // A 16 x 16 fragment at one 4-k step is four instructions q = 0..3: block g
 // computes rows 4 g .. 4 g + 3 against columns 4 q .. 4 q + 3, so the A
@@ -489,6 +490,7 @@ struct Cfg {
   int tm, tn, threads;
   GemmFn fn[4];  // [ta*2+tb]
   int items_per_wg = 1;
+  bool pipelined = false;  // k_gemm_fp64_pipe: the only candidates for blocks of 256 and more
 };
 
 #define SA_CFG(NAME, WM, WN, FM, FN, KT)                                                           \
@@ -519,13 +521,14 @@ struct Cfg {
         {k_gemm_fp64_pipe<WM, WN, FM, FN, KT, false, false, true>,                                        \
          k_gemm_fp64_pipe<WM, WN, FM, FN, KT, false, true, true>,                                         \
          k_gemm_fp64_pipe<WM, WN, FM, FN, KT, true, false, true>, k_gemm_fp64_pipe<WM, WN, FM, FN, KT, true, true, true>}, \
-        1                                                                                                 \
+        1, true                                                                                           \
   }
 #define SA_PCFG(NAME, WM, WN, FM, FN, KT)                                                            \
   Cfg {                                                                                              \
     NAME, 16 * FM * WM, 16 * FN * WN, 64 * WM * WN,                                                  \
         {k_gemm_fp64_pipe<WM, WN, FM, FN, KT, false, false>, k_gemm_fp64_pipe<WM, WN, FM, FN, KT, false, true>, \
-         k_gemm_fp64_pipe<WM, WN, FM, FN, KT, true, false>, k_gemm_fp64_pipe<WM, WN, FM, FN, KT, true, true>}  \
+         k_gemm_fp64_pipe<WM, WN, FM, FN, KT, true, false>, k_gemm_fp64_pipe<WM, WN, FM, FN, KT, true, true>}, \
+        1, true                                                                                      \
   }
 
 static const Cfg kCfgs[] = {
@@ -565,7 +568,7 @@ const char* gemm_fp64_config_name(int cfg) { return (cfg >= 0 && cfg < kNumCfgs)
 
 static int heuristic_cfg(const GemmBatch& b) {
   const int mx = std::max(b.max_m, b.max_n);
-  if (mx > 96) return 1;
+  if (mx > 96) return 19;  // q4pipe128x64_k16
   if (mx > 48) return 0;
   if (mx > 32) return 3;
   if (mx > 16) return 2;
@@ -645,10 +648,12 @@ static int autotune(sa_context* c, const GemmBatch& b) {
   // outputs) -- small-block batches need samples of a few hundred
   // microseconds before the timings rank the configs as the full batch does
   const double per_out = 2.0 * b.max_m * b.max_n * (double)b.max_k * ((double)b.nprods / (double)b.nouts);
-  const int64_t S = std::max<int64_t>(1, std::min<int64_t>(
-                        b.nouts, std::min<int64_t>(32768, std::max<int64_t>(64, (int64_t)(1.6e10 / std::max(per_out, 1.0))))));
-  const int64_t stride = b.nouts / S;
+  const int64_t S0 = std::max<int64_t>(1, std::min<int64_t>(
+                         b.nouts, std::min<int64_t>(32768, std::max<int64_t>(64, (int64_t)(1.6e10 / std::max(per_out, 1.0))))));
   const int64_t slot = (int64_t)b.max_m * b.max_n;
+  // at most 512 MiB of redirected outputs (large dense shapes: a few outputs)
+  const int64_t S = std::max<int64_t>(1, std::min<int64_t>(S0, std::max<int64_t>(4, ((int64_t)1 << 26) / slot)));
+  const int64_t stride = b.nouts / S;
   DevBuf& to = c->buf("tune_outs");
   DevBuf& tc = c->buf("tune_c");
   to.reserve(sizeof(GemmOut) * (size_t)(S + 1));
@@ -668,6 +673,7 @@ static int autotune(sa_context* c, const GemmBatch& b) {
     const Cfg& k = kCfgs[cfg];
     if (cfg != best && ((int64_t)k.tm * k.tn > 4LL * pm * pn || k.tm > 2 * pm + 16 || k.tn > 2 * pn + 16)) continue;
     if (cfg != best && (int64_t)k.tm * k.tn * 16 < std::min<int64_t>((int64_t)pm * pn, 128 * 128)) continue;  // tiny tiles
+    if (cfg != best && pm >= 256 && pn >= 256 && !k.pipelined) continue;  // large blocks: pipelined tiles only
     launch_cfg(c, bs, cfg);  // warm-up (item lists, caches)
     float t = 1e30f;
     for (int rep = 0; rep < 2; ++rep) {

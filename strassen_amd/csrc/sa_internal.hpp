@@ -359,10 +359,6 @@ constexpr int kMlpParams = 32 * kMlpFeatures + 32 + 32 * 32 + 32 + kMlpOutputs *
 bool mlp_loaded(sa_context* c);
 // Y (n x kMlpOutputs) = network(X (n x kMlpFeatures)), device arrays, async
 void mlp_forward_device(sa_context* c, const double* dX, int64_t n, double* dY);
-// rocBLAS on the context stream: C_i = A_i B_i (column-major, no transposes)
-void device_dgemm_strided_batched(sa_context* c, int64_t m, int64_t n, int64_t k, const double* A, int64_t lda,
-                                  int64_t sA, const double* B, int64_t ldb, int64_t sB, double* Cm, int64_t ldc,
-                                  int64_t sC, int64_t count);
 // Tile configurations of the FP64 MFMA kernel.
 int gemm_fp64_num_configs();
 const char* gemm_fp64_config_name(int cfg);

@@ -629,8 +629,10 @@ static void run_plan(sa_bsm* C, sa_bsm* A, sa_bsm* B, Plan& p, int tile_cfg) {
 // A dense root at level l covers w = 2^(l+1) blocks per side: its A region
 // (rows ib0.., columns kb0..) and B region (rows kb0.., columns jb0..) are
 // packed column-major into W x W arrays (W = w bs, absent blocks as zeros),
-// multiplied by one strided-batched library GEMM per batch of equal-level roots,
-// and the W x W products are added into the present C blocks of the region.
+// multiplied by the library's grouped FP64 MFMA GEMM (one output record and one
+// product per root, tile autotuned for the W x W x W shape) per batch of
+// equal-level roots, and the W x W products are added into the present C
+// blocks of the region.
 // Roots sharing a C region are added in ascending kb0 order, one launch per
 // rank, so every C block sums its contributions in a fixed order.
 __global__ void k_pack_regions(const int64_t* __restrict__ off, const double* __restrict__ arena, int32_t nrb,
@@ -734,7 +736,20 @@ static int64_t run_dense_level(sa_bsm* C, sa_bsm* A, sa_bsm* B, std::vector<Root
                        (const double*)A->data(), A->nrb, dai, dak, w, bs, PA);
     hipLaunchKernelGGL(k_pack_regions, dim3(nblk), dim3(256), 0, c->stream, (const int64_t*)B->d_off,
                        (const double*)B->data(), B->nrb, dbk, dbj, w, bs, PB);
-    device_dgemm_strided_batched(c, W, W, W, PA, W, W2, PB, W, W2, PC, W, W2, (int64_t)n);
+    std::vector<GemmOut> ho(n);
+    std::vector<GemmProd> hp(n);
+    for (size_t q = 0; q < n; ++q) {
+      ho[q] = GemmOut{PC + (int64_t)q * W2, (int32_t)W, (int32_t)W, (int32_t)q, (int32_t)q + 1, 0, 0};
+      hp[q] = GemmProd{PA + (int64_t)q * W2, PB + (int64_t)q * W2, (int32_t)W, 0};
+    }
+    GemmBatch gb;
+    gb.outs = upload(c, "dn_outs", ho);
+    gb.nouts = (int64_t)n;
+    gb.prods = upload(c, "dn_prods", hp);
+    gb.nprods = (int64_t)n;
+    gb.max_m = gb.max_n = gb.max_k = (int32_t)W;
+    gb.flops = 2.0 * (double)W * (double)W * (double)W * (double)n;
+    gemm_fp64_run(c, gb, -1);
     for (size_t rk = 0; rk < ranks.size(); ++rk) {
       std::string nm = "dn_sel" + std::to_string(rk);
       const int32_t* dsel = upload(c, nm.c_str(), ranks[rk]);

@@ -8,8 +8,8 @@ run through the hybrid multiply with a caller plan:
   screened-sparse  no root (the screened grouped GEMM of all triples),
   Strassen         every level-l node a Strassen-Winograd root (children
                    recurse down to the cutoff),
-  dense            every level-l node a dense root (packed regions, one batched
-                   library GEMM per chunk of roots);
+  dense            every level-l node a dense root (packed regions, one grouped
+                   FP64 MFMA GEMM launch per chunk of roots);
 
 so each path's per-node cost is amortised over m^3 nodes as it is when a plan
 runs.  The median device time of each (after a warm-up) gives the targets

@@ -182,10 +182,11 @@ def test_fallback_without_model_and_judge():
 
 
 def test_trained_model_gain_over_heuristic():
-    """The committed model (trained on the method's own timings) against the
-    cost-model tree on a banded n=8192 problem of 128-blocks: the learned plan
-    must not be slower (median of 3, 10 % allowance for timing noise); the
-    measured gain is printed."""
+    """The committed model (trained on the method's own timings, dense roots
+    on the library's grouped FP64 kernel) against the cost-model tree on a
+    banded n=8192 problem of 128-blocks: the learned plan must be at least
+    1.2x faster (median of 3 runs each; measured 1.4-2.0x); the gain is
+    printed."""
     import os
     if not os.path.exists(sa.DEFAULT_PATH_MODEL):
         pytest.skip("no trained path model in the tree")
@@ -207,5 +208,5 @@ def test_trained_model_gain_over_heuristic():
         times[name] = float(np.median(ts))
     print(f"learned {times['learned']:.2f} ms vs heuristic {times['heuristic']:.2f} ms "
           f"(gain {times['heuristic'] / times['learned']:.3f}x)")
-    assert times["learned"] <= 1.1 * times["heuristic"]
+    assert times["heuristic"] >= 1.2 * times["learned"]
 
