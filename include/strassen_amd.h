@@ -93,6 +93,12 @@ int sa_ctx_gemm_config(sa_ctx_t ctx, int32_t* cfg, char* name, int32_t name_len)
  * (default 160, 0 = never).  No counterpart in the reference, which runs
  * per-block dgemm calls (blocksparsematrix.hpp:363-405). */
 int sa_ctx_kwindows(sa_ctx_t ctx, int32_t* nwin);
+/* HIP-graph counters of the context: executables instantiated, executables
+ * updated in place from a new capture (replays of the same topology) and
+ * graph launches.  The Strassen execution phase of sa_matmult_strassen_sparse is
+ * captured on every call and replayed through one kept executable.  No
+ * counterpart in the reference, which has no device graphs. */
+int sa_ctx_graph_stats(sa_ctx_t ctx, int64_t* instantiated, int64_t* updated, int64_t* launched);
 /* Wall time of the context stream between two sa_ctx_mark calls (ms). */
 int sa_ctx_mark(sa_ctx_t ctx, int slot);
 int sa_ctx_elapsed(sa_ctx_t ctx, int slot0, int slot1, float* ms);

@@ -64,6 +64,7 @@ ABI = {
     "sa_ctx_profile_reset": (INT, [P]),
     "sa_ctx_gemm_config": (INT, [P, P, C.c_char_p, I32]),
     "sa_ctx_kwindows": (INT, [P, P]),
+    "sa_ctx_graph_stats": (INT, [P, P, P, P]),
     "sa_mlp_info": (INT, [P, P, P]),
     "sa_mlp_set": (INT, [P, P, I64, P, P]),
     "sa_mlp_get": (INT, [P, P, P, P]),
@@ -206,6 +207,12 @@ class Context:
         n = I32(0)
         _check(lib().sa_ctx_kwindows(self.h, C.byref(n)))
         return n.value
+
+    def graph_stats(self) -> dict:
+        """HIP-graph counters: executables instantiated, updated in place, launches."""
+        v = [C.c_int64(0) for _ in range(3)]
+        _check(lib().sa_ctx_graph_stats(self.h, *(C.byref(x) for x in v)))
+        return dict(zip(("instantiated", "updated", "launched"), (x.value for x in v)))
 
     def mark(self, slot: int):
         _check(lib().sa_ctx_mark(self.h, slot))

@@ -21,8 +21,10 @@ void set_error(const char* fmt, ...) {
 void DevBuf::reserve(size_t n) {
   if (n <= bytes) return;
   if (p) {
-    // kernels queued earlier may still read the old buffer
-    SA_CHECK_HIP(hipDeviceSynchronize());
+    // kernels queued earlier on the owning stream may still read the old
+    // buffer (a buffer without an owner waits for the whole device)
+    if (stream) SA_CHECK_HIP(hipStreamSynchronize(stream));
+    else SA_CHECK_HIP(hipDeviceSynchronize());
     SA_CHECK_HIP(hipFree(p));
   }
   p = nullptr;
@@ -298,25 +300,44 @@ void run_as_graph(sa_context* c, const std::function<void()>& f) {
     f();
     return;
   }
-  hipGraphExec_t e = nullptr;
-  const hipError_t ie = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
-  if (ie != hipSuccess) {
-    graph_note("graph instantiation failed", ie);
-    (void)hipGetLastError();
-    (void)hipGraphDestroy(g);
-    f();  // the captured launches did not run: run them eagerly
-    return;
+  // replay: the kept executable takes this capture's kernel parameters when
+  // the captured topology is the same; otherwise it is instantiated anew
+  bool ready = false;
+  if (c->graph_exec) {
+    hipGraphNode_t bad = nullptr;
+    hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+    const hipError_t ue = hipGraphExecUpdate(c->graph_exec, g, &bad, &res);
+    if (ue == hipSuccess && res == hipGraphExecUpdateSuccess) {
+      ready = true;
+      ++c->graph_updated;
+    } else {
+      (void)hipGetLastError();
+      (void)hipGraphExecDestroy(c->graph_exec);
+      c->graph_exec = nullptr;
+    }
+  }
+  if (!ready) {
+    const hipError_t ie = hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0);
+    if (ie != hipSuccess) {
+      graph_note("graph instantiation failed", ie);
+      (void)hipGetLastError();
+      c->graph_exec = nullptr;
+      (void)hipGraphDestroy(g);
+      f();  // the captured launches did not run: run them eagerly
+      return;
+    }
+    ++c->graph_instantiated;
   }
   hipError_t le;
   {
     ProfScope ps(c, "graph");
-    le = hipGraphLaunch(e, c->stream);
+    le = hipGraphLaunch(c->graph_exec, c->stream);
   }
   const hipError_t se = hipStreamSynchronize(c->stream);
-  (void)hipGraphExecDestroy(e);
   (void)hipGraphDestroy(g);
   SA_CHECK_HIP(le);
   SA_CHECK_HIP(se);
+  ++c->graph_launched;
 }
 
 int64_t read_i64(sa_context* c, const int64_t* d) {
@@ -365,6 +386,9 @@ int sa_ctx_create(int device, sa_ctx_t* out) {
     auto* c = new sa_context();
     c->device = device;
     SA_CHECK_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    for (sa::DevBuf* b : {&c->ws_scan, &c->ws_tmp0, &c->ws_tmp1, &c->ws_tmp2, &c->ws_stage, &c->ws_gemm_items,
+                          &c->ws_gemm_items2})
+      b->stream = c->stream;
     hipDeviceProp_t prop;
     SA_CHECK_HIP(hipGetDeviceProperties(&prop, device));
     c->num_cus = prop.multiProcessorCount;
@@ -384,6 +408,7 @@ int sa_ctx_destroy(sa_ctx_t c) {
       (void)hipEventDestroy(p.second.second);
     }
     for (auto& e : c->marks) (void)hipEventDestroy(e);
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
     c->ws_scan.release(); c->ws_tmp0.release(); c->ws_tmp1.release(); c->ws_tmp2.release();
     c->ws_stage.release(); c->ws_gemm_items.release(); c->ws_gemm_items2.release();
     for (auto& kv : c->bufs) kv.second.release();
@@ -435,6 +460,15 @@ int sa_ctx_gemm_config(sa_ctx_t c, int32_t* cfg, char* name, int32_t name_len) {
     if (name && name_len > 0) {
       std::snprintf(name, (size_t)name_len, "%s", c->last_gemm_cfg >= 0 ? gemm_fp64_config_name(c->last_gemm_cfg) : "");
     }
+  });
+}
+
+int sa_ctx_graph_stats(sa_ctx_t c, int64_t* instantiated, int64_t* updated, int64_t* launched) {
+  return guard([&] {
+    SA_REQUIRE(c, "null context");
+    if (instantiated) *instantiated = c->graph_instantiated;
+    if (updated) *updated = c->graph_updated;
+    if (launched) *launched = c->graph_launched;
   });
 }
 

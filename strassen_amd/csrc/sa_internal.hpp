@@ -72,6 +72,7 @@ int guard(F&& f) {
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  hipStream_t stream = nullptr;      // the owning context's stream (the only one that uses the buffer)
   void reserve(size_t n);            // grow-only, contents discarded
   void reserve_keep(size_t n, hipStream_t s);  // grow-only, contents kept
   void release();
@@ -186,10 +187,19 @@ struct sa_context {
   bool quiet = false;     // no profiling events (allocation-free calls)
   int32_t last_gemm_cfg = -1;
   int32_t last_kwindows = 1;  // k-windows of the last screened multiply
+  // executable graph of the Strassen execution phase, kept between calls:
+  // each call's capture is applied to it (hipGraphExecUpdate) and it is
+  // launched again; a capture of another shape instantiates it anew
+  hipGraphExec_t graph_exec = nullptr;
+  int64_t graph_instantiated = 0, graph_updated = 0, graph_launched = 0;
   // learned path selector (device copy of the network in buffer "mlp_prm")
   bool mlp_loaded = false;
   int32_t mlp_step = 0;  // Adam steps taken
-  sa::DevBuf& buf(const char* name) { return bufs[name]; }
+  sa::DevBuf& buf(const char* name) {
+    sa::DevBuf& b = bufs[name];
+    b.stream = stream;
+    return b;
+  }
   // profiling
   bool profiling = false;
   std::map<std::string, sa::ProfAcc> prof;

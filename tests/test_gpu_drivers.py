@@ -23,8 +23,10 @@ def run(args, timeout=300):
 def test_occ_ri_k_step_h2o():
     out = run([os.path.join(BIN, "occ_ri_k_step"), "24", "113", "5", "322", "3c_ints_PQm12_h2o.data",
                "occ_mo_h2o.data", "v2m_key_h2o.data", "3", "1e-20"])
-    dense = float(re.search(r"L2 norm of output = (\S+)", out).group(1))
-    sparse = float(re.search(r"L2 norm of sparse output = (\S+)", out).group(1))
+    # the reference's own lines (occ_ri_k_step.cpp:70-130), formats unchanged
+    assert re.search(r"L2 norm of output = \d\.\d{6}e\+01\n", out), out
+    assert len(re.findall(r"  \[\d\] \d+\.\d{4} s  \(\d+\.\d{4} GFLOPs\)", out)) == 6, out
+    dense, sparse = (float(x) for x in re.search(r"full-precision L2 norms: dense (\S+) sparse (\S+)", out).groups())
     assert abs(dense - 10.415071322740426) < 1e-12, out
     assert abs(sparse - 10.415071322740426) < 1e-12, out
     for v in re.findall(r"relative RMSD = (\S+)", out):

@@ -117,6 +117,33 @@ def test_levels_run_as_one_graph_repeatedly():
         c.set_profiling(False)
 
 
+def test_graph_executable_replayed():
+    """One executable graph is kept per context: a call of the same shape
+    updates it in place from the new capture and launches it again, a call of
+    another shape instantiates it anew; every replay is exact on new integer
+    inputs, with eager screened multiplies between the replays."""
+    c = ctx()
+    s0 = c.graph_stats()
+    shapes = [(16, 4, {1, 2})] * 3 + [(8, 8, {1})] + [(16, 4, {1, 2})] * 2
+    for i, (nb, bs, levels) in enumerate(shapes):
+        rng = np.random.default_rng(200 + i)
+        n, a, b = inputs(rng, nb, bs, "dense", integer=True)
+        co, cd, st = run_both(a, b, n, bs, 0.0, 0, oracle.forced_tree(nb, levels))
+        assert st["nroots"] > 0
+        assert np.array_equal(cd, a @ b), f"call {i}"
+        _, dA = pair_uniform(a, bs, bs, 0.0)
+        _, dB = pair_uniform(b, bs, bs, 0.0)
+        dC = sa.BSM.uniform(c, n, n, bs, bs, 0.0)
+        sa.matmult(dC, dA, 0, dB, 0, 0.0, 1.0, 0.0)
+        assert np.array_equal(dC.to_dense(), a @ b)
+    s1 = c.graph_stats()
+    d = {k: s1[k] - s0[k] for k in s1}
+    assert d["launched"] == len(shapes)
+    assert d["instantiated"] + d["updated"] == len(shapes)
+    assert d["updated"] >= 3, d       # calls 2, 3 and 6 replay the kept executable
+    assert d["instantiated"] >= 2, d  # the other shape, and the way back
+
+
 def random_plan(rng, nb, p_s, p_d):
     sizes = oracle.tree_level_sizes(nb)
     return ([(rng.random(s) < p_s).astype(np.uint8) for s in sizes],
