@@ -608,6 +608,46 @@ void gemm_fp64_launch(sa_context* c, const GemmBatch& b, const GemmPlan& g) {
 
 static void launch_cfg(sa_context* c, const GemmBatch& b, int cfg) { gemm_fp64_launch(c, b, plan_items(c, b, cfg)); }
 
+// ------------------------------------------------------- window items --
+__global__ void k_gather_at(const int64_t* __restrict__ src, const int64_t* __restrict__ idx, int64_t n,
+                            int64_t* __restrict__ dst) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[idx[i]];
+}
+
+WindowItems window_items(sa_context* c, const GemmOut* outs, const std::vector<int64_t>& off, int TM, int TN,
+                         int32_t max_m, int32_t max_n) {
+  WindowItems W;
+  W.off = off;
+  const int64_t n = off.empty() ? 0 : off.back();
+  if (n == 0 || (max_m <= TM && max_n <= TN)) return W;
+  const int64_t nw1 = (int64_t)off.size();
+  DevBuf& bc = c->buf("win_cnt");
+  DevBuf& bi = c->buf("win_idx");
+  bc.reserve(sizeof(int64_t) * (size_t)(n + 1));
+  bi.reserve(sizeof(int64_t) * (size_t)(2 * nw1));
+  int64_t* cnt = bc.as<int64_t>();
+  int64_t* idx = bi.as<int64_t>();
+  SA_CHECK_HIP(hipMemcpyAsync(idx, off.data(), sizeof(int64_t) * (size_t)nw1, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_count_tiles, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, c->stream, outs, n, TM, TN, cnt);
+  scan_exclusive_i64_async(c, cnt, n, cnt + n);  // cnt[n]: the total
+  hipLaunchKernelGGL(k_gather_at, dim3((unsigned)ceil_div(nw1, 256)), dim3(256), 0, c->stream, (const int64_t*)cnt,
+                     (const int64_t*)idx, nw1, idx + nw1);
+  int64_t* h = static_cast<int64_t*>(pinned(c, sizeof(int64_t) * (size_t)nw1));
+  SA_CHECK_HIP(hipMemcpyAsync(h, idx + nw1, sizeof(int64_t) * (size_t)nw1, hipMemcpyDeviceToHost, c->stream));
+  SA_CHECK_HIP(hipStreamSynchronize(c->stream));
+  W.off.assign(h, h + nw1);
+  const int64_t nitems = W.off.back();
+  SA_REQUIRE(n < ((int64_t)1 << 31) && nitems < ((int64_t)1 << 31), "too many GEMM work items");
+  DevBuf& bit = c->buf("win_items");
+  bit.reserve(sizeof(GemmItem) * (size_t)(nitems + 1));
+  hipLaunchKernelGGL(k_emit_items, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, c->stream, outs, n, TM, TN,
+                     (const int64_t*)cnt, bit.as<GemmItem>());
+  SA_CHECK_HIP(hipGetLastError());
+  W.items = bit.as<GemmItem>();
+  return W;
+}
+
 // ---------------------------------------------------------------- autotuner --
 // The tile configuration is chosen per shape class (transposes, largest block
 // dimensions) by timing every plausible configuration on a sample of the real
@@ -723,6 +763,15 @@ GemmPlan gemm_fp64_plan_fixed(const GemmBatch& b, int cfg, TableMem& mem) {
     items = it;
   }
   return GemmPlan{cfg, items, nitems};
+}
+
+int gemm_fp64_choose(sa_context* c, const GemmBatch& b, int cfg) {
+  return (cfg >= 0 && cfg < kNumCfgs) ? cfg : autotune(c, b);
+}
+
+void gemm_fp64_tile(int cfg, int* tm, int* tn) {
+  *tm = kCfgs[cfg].tm;
+  *tn = kCfgs[cfg].tn;
 }
 
 GemmPlan gemm_fp64_plan(sa_context* c, const GemmBatch& b, int cfg) {

@@ -387,6 +387,22 @@ GemmPlan gemm_fp64_plan(sa_context* c, const GemmBatch& b, int cfg);
 // cached choice for this shape class, else the size heuristic; work items in mem
 GemmPlan gemm_fp64_plan_fixed(const GemmBatch& b, int cfg, TableMem& mem);
 void gemm_fp64_launch(sa_context* c, const GemmBatch& b, const GemmPlan& g);
+// tile config id for a batch: cfg when valid, else the autotuner's choice
+int gemm_fp64_choose(sa_context* c, const GemmBatch& b, int cfg);
+// output tile (rows, columns) of a tile config
+void gemm_fp64_tile(int cfg, int* tm, int* tn);
+// Work items of every k-window at once: the tiles of the window-ordered
+// outputs (TM x TN tiles; item.out indexes outs) counted, scanned and emitted
+// in one pass and split at the window boundaries off, with one host read-back
+// of the nwin + 1 item offsets, so the windows' launches then follow each
+// other without host synchronisation.  items == nullptr when no block exceeds
+// one tile (then window w's items are its outputs off[w] .. off[w + 1]).
+struct WindowItems {
+  const GemmItem* items = nullptr;
+  std::vector<int64_t> off;  // nwin + 1 item offsets (outputs when items == nullptr)
+};
+WindowItems window_items(sa_context* c, const GemmOut* outs, const std::vector<int64_t>& off, int TM, int TN,
+                         int32_t max_m, int32_t max_n);
 // run f's launches on c's stream as one HIP graph (captured, instantiated,
 // launched once); f must not synchronise or copy from the host
 void run_as_graph(sa_context* c, const std::function<void()>& f);

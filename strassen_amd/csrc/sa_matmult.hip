@@ -446,11 +446,40 @@ static int64_t screened_multiply_mem(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, b
   } else {
     const KWindows W = plan_kwindows(c, A, ta, B, tb, R, outs);
     c->last_kwindows = W.nwin;
-    for (int w = 0; w < W.nwin; ++w) {
-      GemmBatch gw = gb2;
-      gw.outs = W.outs + W.off[(size_t)w];
-      gw.nouts = W.off[(size_t)w + 1] - W.off[(size_t)w];
-      if (gw.nouts > 0) gemm_fp64_run(c, gw, tile_cfg);
+    if (W.nwin == 1) {
+      gemm_fp64_run(c, gb2, tile_cfg);
+    } else {
+      // one tile config (the shape class is the whole multiply's) and the
+      // work items of all windows planned together; the launches then run
+      // back to back in ascending kb
+      GemmBatch g0 = gb2;
+      for (int w = 0; w < W.nwin; ++w)
+        if (W.off[(size_t)w + 1] > W.off[(size_t)w]) {
+          g0.outs = W.outs + W.off[(size_t)w];
+          g0.nouts = W.off[(size_t)w + 1] - W.off[(size_t)w];
+          break;
+        }
+      const int cfg = gemm_fp64_choose(c, g0, tile_cfg);
+      c->last_gemm_cfg = cfg;
+      int TM = 0, TN = 0;
+      gemm_fp64_tile(cfg, &TM, &TN);
+      const WindowItems wi = window_items(c, W.outs, W.off, TM, TN, mm, mn);
+      for (int w = 0; w < W.nwin; ++w) {
+        GemmBatch gw = gb2;
+        GemmPlan g;
+        g.cfg = cfg;
+        g.nitems = wi.off[(size_t)w + 1] - wi.off[(size_t)w];
+        if (g.nitems == 0) continue;
+        if (wi.items) {
+          gw.outs = W.outs;
+          g.items = wi.items + wi.off[(size_t)w];
+        } else {
+          gw.outs = W.outs + W.off[(size_t)w];
+        }
+        gw.nouts = W.off[(size_t)w + 1] - W.off[(size_t)w];
+        ProfScope ps(c, "gemm_fp64");
+        gemm_fp64_launch(c, gw, g);
+      }
     }
   }
   mirror_lower(C, R);

@@ -872,23 +872,6 @@ static const OzCfg kOzCfgs[] = {
     SA_OZW(4, 2, 2, 4, 1.2), SA_OZW(4, 2, 4, 2, 1.2),            // 34-35: INT8 S4 64x128 / 128x64, wide
 };
 
-__global__ void k_count_tiles_oz(const GemmOut* __restrict__ outs, int64_t nouts, int TM, int TN,
-                                 int64_t* __restrict__ cnt) {
-  const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (o >= nouts) return;
-  cnt[o] = ceil_div(outs[o].m, TM) * ceil_div(outs[o].n, TN);
-}
-
-__global__ void k_emit_items_oz(const GemmOut* __restrict__ outs, int64_t nouts, int TM, int TN,
-                                const int64_t* __restrict__ start, GemmItem* __restrict__ items) {
-  const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (o >= nouts) return;
-  const int tm = (int)ceil_div(outs[o].m, TM), tn = (int)ceil_div(outs[o].n, TN);
-  int64_t s = start[o];
-  for (int b = 0; b < tn; ++b)
-    for (int a = 0; a < tm; ++a) items[s++] = GemmItem{(int32_t)o, (int16_t)a, (int16_t)b};
-}
-
 // slice one operand (all present blocks); returns device pointers
 struct Sliced {
   const uint8_t* sl;
@@ -1042,24 +1025,14 @@ int64_t ozaki_multiply(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, double
   const int TM = 32 * k.fm, TN = 32 * k.fn;
   const KWindows W = plan_kwindows(c, A, ta, B, tb, R, bouts.as<GemmOut>());
   c->last_kwindows = W.nwin;
+  // work items of all windows planned together (one read-back); the window
+  // launches then run back to back in ascending kb
+  const WindowItems wi = window_items(c, W.outs, W.off, TM, TN, mm, mn);
   for (int w = 0; w < W.nwin; ++w) {
-    const GemmOut* wo = W.outs + W.off[(size_t)w];
-    const int64_t nw = W.off[(size_t)w + 1] - W.off[(size_t)w];
-    if (nw == 0) continue;
-    const GemmItem* items = nullptr;
-    int64_t nitems = nw;
-    if (mm > TM || mn > TN) {
-      c->ws_gemm_items.reserve(sizeof(int64_t) * (size_t)(nw + 1));
-      int64_t* cnt = c->ws_gemm_items.as<int64_t>();
-      hipLaunchKernelGGL(k_count_tiles_oz, dim3((unsigned)ceil_div(nw, 256)), dim3(256), 0, c->stream, wo, nw, TM,
-                         TN, cnt);
-      nitems = scan_exclusive_i64(c, cnt, nw);
-      c->ws_gemm_items2.reserve(sizeof(GemmItem) * (size_t)(nitems + 1));
-      hipLaunchKernelGGL(k_emit_items_oz, dim3((unsigned)ceil_div(nw, 256)), dim3(256), 0, c->stream, wo, nw, TM,
-                         TN, (const int64_t*)cnt, c->ws_gemm_items2.as<GemmItem>());
-      items = c->ws_gemm_items2.as<GemmItem>();
-    }
-    SA_REQUIRE(nitems < ((int64_t)1 << 31), "ozaki: too many work items");
+    const int64_t nitems = wi.off[(size_t)w + 1] - wi.off[(size_t)w];
+    if (nitems == 0) continue;
+    const GemmOut* wo = wi.items ? W.outs : W.outs + W.off[(size_t)w];
+    const GemmItem* items = wi.items ? wi.items + wi.off[(size_t)w] : nullptr;
     ProfScope ps(c, "gemm_ozaki");
     hipLaunchKernelGGL(k.fn_, dim3((unsigned)nitems), dim3((unsigned)k.threads), 0, c->stream, wo,
                        (const OzProd*)bprods.as<OzProd>(), items, nitems, alpha);
