@@ -18,6 +18,7 @@
 #include <map>
 #include <string>
 #include <functional>
+#include <memory>
 #include <vector>
 
 #include "../../include/strassen_amd.h"
@@ -339,8 +340,18 @@ ScreenResult screen_and_allocate(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool 
 ScreenResult screen_and_allocate(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, double thresh,
                                  const uint8_t* d_tree, const int64_t* d_lvl_off, int nlev, bool c_must_exist);
 void mirror_lower(sa_bsm* C, const ScreenResult& R);
+// A split-precision operand prepared ahead of its multiply (sa_ozaki.hip).
+struct OzPrep;
+struct OzPrepDelete {
+  void operator()(OzPrep* p) const;
+};
+using OzPrepHandle = std::unique_ptr<OzPrep, OzPrepDelete>;
+// For op(A) = V^T, V a reshape view of M (view block b = M's block b): M's
+// block norms and the slicer's exponents of V's columns in one pass over M;
+// ozaki_multiply(C, V, true, ..., prep) then only cuts the digits.
+OzPrepHandle ozaki_prepare_view(sa_bsm* M, sa_bsm* V, const sa_opts_t& o);
 int64_t ozaki_multiply(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, double thresh, double alpha,
-                       const sa_opts_t& o, bool sym = false);
+                       const sa_opts_t& o, bool sym = false, const OzPrep* prep = nullptr);
 int64_t screened_multiply(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, double thresh, double alpha,
                           int tile_cfg, const uint8_t* d_tree, const int64_t* d_lvl_off, int nlev, bool c_must_exist,
                           bool sym = false);

@@ -152,6 +152,25 @@ __global__ void k_slice_need(SliceArgs s, int S, int kstep, int esize, int64_t* 
 // exponents already in exps (shared by all blocks of a vector's line)
 constexpr int kNoExp = INT32_MIN / 2;  // exponent of an all-zero vector before sharing
 
+// Scale exponent e of a vector whose largest magnitude is mx: mx < 2^e.  INT8:
+// one guard bit of the leading digit is needed only when the vector's largest
+// value would carry it to +128; otherwise the scale moves up one bit (every
+// |x| <= mx then gives a leading digit in [-128, 127]), which halves the
+// truncation of the vector's digits.  An all-zero vector gets `none`.
+template <int KIND>
+__device__ inline int oz_vector_exp(double mx, int S, int none) {
+  constexpr int BETA = oz_beta(KIND);
+  if (!(mx > 0.0)) return none;
+  int e = 0;
+  (void)frexp(mx, &e);  // mx < 2^e
+  if (KIND == 2) {
+    int64_t H = 0;
+    for (int j = 0; j < S - 1; ++j) H += ((int64_t)1 << (BETA - 1)) << (BETA * j);
+    if ((int64_t)rint(ldexp(mx, BETA * S - 1 - e)) + H < ((int64_t)1 << (BETA * S - 1))) e -= 1;
+  }
+  return e;
+}
+
 template <int KIND>
 __global__ void __launch_bounds__(64) k_slice(SliceArgs s, int S, const int64_t* __restrict__ sl_off,
                                               const int64_t* __restrict__ ex_off, int64_t ngroups,
@@ -220,16 +239,8 @@ __global__ void __launch_bounds__(64) k_slice(SliceArgs s, int S, const int64_t*
     }
     mx = fmax(mx, __shfl_xor(mx, 16, 64));
     mx = fmax(mx, __shfl_xor(mx, 32, 64));
-    if (mx > 0.0) {
-      (void)frexp(mx, &e);  // mx < 2^e
-      // INT8: one guard bit of the leading digit is needed only when the
-      // vector's largest value would carry it to +128; otherwise the scale
-      // moves up one bit (every |x| <= mx then gives a leading digit in
-      // [-128, 127]), which halves the truncation of the vector's digits
-      if (KIND == 2 && (int64_t)rint(ldexp(mx, BETA * S - 1 - e)) + H < ((int64_t)1 << (BETA * S - 1))) e -= 1;
-    } else if (mode == 1) {
-      e = kNoExp;  // a zero vector does not raise its line's exponent
-    }
+    // a zero vector does not raise its line's exponent (mode 1)
+    e = oz_vector_exp<KIND>(mx, S, mode == 1 ? kNoExp : 0);
     if (kg == 0) exps[ex_off[b] + v0 + vl] = e;
     if (mode == 1) return;
   }
@@ -257,6 +268,65 @@ __global__ void __launch_bounds__(64) k_slice(SliceArgs s, int S, const int64_t*
       }
       *reinterpret_cast<uint4*>(out + (int64_t)p * pstride + (int64_t)ks * 1024) = make_uint4(w[0], w[1], w[2], w[3]);
     }
+  }
+}
+
+// One pass over every block of a matrix M whose slicing view V (same blocks,
+// same memory, another block geometry: view block b holds M's block b) takes
+// V's columns as op(A) rows of a transposed operand, i.e. vectors of
+// k-contiguous elements: element e of a block (flat, column-major) is k = e %
+// L of vector e / L, L = V's block rows.  One wave per block computes
+//  * M's canonical block norm (the order of wave_block_norm: lane l sums
+//    fma(x, x) over e = l, l + 64, ..., then the halving tree), and
+//  * each vector's largest magnitude, and from it the slicer's exponent
+//    (oz_vector_exp), written where k_slice mode 2 reads it,
+// so the slicer's own max pass over the operand is not needed.  Padding
+// vectors (up to a multiple of 16) get the zero-vector exponent.
+template <int KIND>
+__global__ void __launch_bounds__(64) k_norms_view_exps(const double* __restrict__ arena,
+                                                        const int64_t* __restrict__ off, const int32_t* __restrict__ mrsz,
+                                                        const int32_t* __restrict__ mcsz, int32_t mnrb,
+                                                        const int32_t* __restrict__ vrsz,
+                                                        const int32_t* __restrict__ vcsz, int32_t vnrb, int64_t nb,
+                                                        int S, int none, const int64_t* __restrict__ ex_off,
+                                                        double* __restrict__ norm, int32_t* __restrict__ exps) {
+  __shared__ unsigned long long colmax[128];  // |x| bits: non-negative doubles order as integers
+  const int lane = threadIdx.x;
+  const int64_t b = blockIdx.x;
+  if (b >= nb) return;
+  const int64_t o = off[b];
+  if (o < 0) {
+    if (lane == 0) norm[b] = 0.0;
+    return;
+  }
+  const int64_t n = (int64_t)mrsz[b % mnrb] * mcsz[b / mnrb];
+  const int L = vrsz[b % vnrb], nv = vcsz[b / vnrb];
+  const int nvp = (int)ceil_div(nv, 16) * 16;
+  for (int v = lane; v < 128; v += 64) colmax[v] = 0ull;
+  __syncthreads();
+  const double* x = arena + o;
+  double a = 0.0, mx = 0.0;
+  int v = lane / L;                    // vector of element lane
+  int64_t next = (int64_t)(v + 1) * L;  // first element of the following vector
+  for (int64_t e = lane; e < n; e += 64) {
+    const double y = x[e];
+    a = fma(y, y, a);
+    if (e >= next) {  // crossed into another vector: flush the running max
+      atomicMax(&colmax[v], (unsigned long long)__double_as_longlong(mx));
+      v = (int)(e / L);
+      next = (int64_t)(v + 1) * L;
+      mx = 0.0;
+    }
+    mx = fmax(mx, fabs(y));
+  }
+  if (lane < n) atomicMax(&colmax[v], (unsigned long long)__double_as_longlong(mx));
+#pragma unroll
+  for (int s = 32; s >= 1; s >>= 1) a = a + __shfl_down(a, s, 64);
+  if (lane == 0) norm[b] = sqrt(a);
+  __syncthreads();
+  for (int c = lane; c < nvp; c += 64) {
+    const double m = c < nv ? __longlong_as_double((long long)colmax[c]) : 0.0;
+    exps[ex_off[b] + c] = oz_vector_exp<KIND>(m, S, none);
   }
 }
 
@@ -880,10 +950,10 @@ struct Sliced {
   const int64_t* ex_off;
 };
 
-// shared: exponents per op-vector line across blocks (k_exp_share), else per block
-static Sliced slice_operand(sa_context* c, sa_bsm* X, bool role_b, bool trans, int S, int kind, bool shared,
-                            const char* tag) {
-  const int64_t nb = X->nb();
+// slice sizes and offsets of every block, buffers reserved (one read-back of
+// the totals); the exponents are not written yet
+static Sliced slice_plan(sa_context* c, const SliceArgs& a, int S, int kind, const char* tag, int64_t* nex_out) {
+  const int64_t nb = a.nb;
   std::string t(tag);
   DevBuf& bso = c->buf((t + "_sloff").c_str());
   DevBuf& beo = c->buf((t + "_exoff").c_str());
@@ -891,7 +961,6 @@ static Sliced slice_operand(sa_context* c, sa_bsm* X, bool role_b, bool trans, i
   bso.reserve(sizeof(int64_t) * (size_t)(nb + 1));
   beo.reserve(sizeof(int64_t) * (size_t)(nb + 1));
   btot.reserve(sizeof(int64_t) * 2);
-  SliceArgs a{X->d_off, X->data(), X->d_rsz, X->d_csz, X->nrb, nb, role_b ? 1 : 0, trans ? 1 : 0};
   hipLaunchKernelGGL(k_slice_need, dim3((unsigned)ceil_div(nb, 256)), dim3(256), 0, c->stream, a, S, oz_kstep(kind),
                      oz_esize(kind), bso.as<int64_t>(), beo.as<int64_t>());
   scan_exclusive_i64_async(c, bso.as<int64_t>(), nb, btot.as<int64_t>());
@@ -904,37 +973,102 @@ static Sliced slice_operand(sa_context* c, sa_bsm* X, bool role_b, bool trans, i
   DevBuf& bex = c->buf((t + "_ex").c_str());
   bsl.reserve((size_t)(nsl + 64));
   bex.reserve(sizeof(int32_t) * (size_t)(nex + 1));
-  {
-    ProfScope ps(c, "ozaki_slice");
-    if (nb) {
-      const int64_t ng = nex / 16;  // 16-vector groups (nex: padded vectors of all blocks)
-      const dim3 g((unsigned)std::max<int64_t>(1, ng)), t(64);
-      const int64_t *so = (const int64_t*)bso.p, *eo = (const int64_t*)beo.p;
-      auto run = [&](int mode) {
-        if (kind == 2)
-          hipLaunchKernelGGL(k_slice<2>, g, t, 0, c->stream, a, S, so, eo, ng, bsl.as<uint8_t>(), bex.as<int32_t>(),
-                             mode);
-        else if (kind == 1)
-          hipLaunchKernelGGL(k_slice<1>, g, t, 0, c->stream, a, S, so, eo, ng, bsl.as<uint8_t>(), bex.as<int32_t>(),
-                             mode);
-        else
-          hipLaunchKernelGGL(k_slice<0>, g, t, 0, c->stream, a, S, so, eo, ng, bsl.as<uint8_t>(), bex.as<int32_t>(),
-                             mode);
-      };
-      if (shared) {
-        run(1);
-        const bool by_row = role_b ? trans : !trans;
-        hipLaunchKernelGGL(k_exp_share, dim3((unsigned)(by_row ? X->nrb : X->ncb)), dim3(256), 0, c->stream, a, eo,
-                           bex.as<int32_t>());
-        run(2);
-      } else {
-        run(0);
-      }
-    }
-  }
-  SA_CHECK_HIP(hipGetLastError());
+  *nex_out = nex;
   return Sliced{bsl.as<uint8_t>(), bso.as<int64_t>(), bex.as<int32_t>(), beo.as<int64_t>()};
 }
+
+// cut the digits (and the exponents first, unless exps_ready: then they are
+// in place per block, or per line before sharing when shared)
+static void slice_run(sa_context* c, const SliceArgs& a, int32_t nrb, int32_t ncb, int S, int kind, bool shared,
+                      const Sliced& sl, int64_t nex, bool exps_ready) {
+  ProfScope ps(c, "ozaki_slice");
+  if (a.nb == 0) return;
+  const int64_t ng = nex / 16;  // 16-vector groups (nex: padded vectors of all blocks)
+  const dim3 g((unsigned)std::max<int64_t>(1, ng)), t(64);
+  uint8_t* slp = const_cast<uint8_t*>(sl.sl);
+  int32_t* exp = const_cast<int32_t*>(sl.ex);
+  auto run = [&](int mode) {
+    if (kind == 2)
+      hipLaunchKernelGGL(k_slice<2>, g, t, 0, c->stream, a, S, sl.sl_off, sl.ex_off, ng, slp, exp, mode);
+    else if (kind == 1)
+      hipLaunchKernelGGL(k_slice<1>, g, t, 0, c->stream, a, S, sl.sl_off, sl.ex_off, ng, slp, exp, mode);
+    else
+      hipLaunchKernelGGL(k_slice<0>, g, t, 0, c->stream, a, S, sl.sl_off, sl.ex_off, ng, slp, exp, mode);
+  };
+  if (shared) {
+    if (!exps_ready) run(1);
+    const bool by_row = a.role_b ? a.trans : !a.trans;
+    hipLaunchKernelGGL(k_exp_share, dim3((unsigned)(by_row ? nrb : ncb)), dim3(256), 0, c->stream, a, sl.ex_off, exp);
+    run(2);
+  } else {
+    run(exps_ready ? 2 : 0);
+  }
+  SA_CHECK_HIP(hipGetLastError());
+}
+
+static SliceArgs slice_args(sa_bsm* X, bool role_b, bool trans) {
+  return SliceArgs{X->d_off, X->data(), X->d_rsz, X->d_csz, X->nrb, X->nb(), role_b ? 1 : 0, trans ? 1 : 0};
+}
+
+// shared: exponents per op-vector line across blocks (k_exp_share), else per block
+static Sliced slice_operand(sa_context* c, sa_bsm* X, bool role_b, bool trans, int S, int kind, bool shared,
+                            const char* tag) {
+  const SliceArgs a = slice_args(X, role_b, trans);
+  int64_t nex = 0;
+  const Sliced sl = slice_plan(c, a, S, kind, tag, &nex);
+  slice_run(c, a, X->nrb, X->ncb, S, kind, shared, sl, nex, false);
+  return sl;
+}
+
+struct OzPrep {
+  const sa_bsm* V = nullptr;  // the view the operand slices
+  int S = 0, kind = 0;
+  bool shared = false;
+  Sliced sl{};
+  int64_t nex = 0;
+};
+
+OzPrepHandle ozaki_prepare_view(sa_bsm* M, sa_bsm* V, const sa_opts_t& o) {
+  sa_context* c = M->ctx;
+  SA_REQUIRE(V->nb() == M->nb(), "ozaki: the view must hold the matrix's blocks");
+  int32_t mx = 0;
+  for (int32_t v : V->csz) mx = std::max(mx, v);
+  SA_REQUIRE(mx <= 128, "ozaki: view blocks of at most 128 columns (largest %d)", mx);
+  auto* p = new OzPrep();
+  p->V = V;
+  p->S = o.slices;
+  p->kind = o.slice_type;
+  p->shared = o.shared_exp != 0;
+  const SliceArgs a = slice_args(V, false, true);
+  p->sl = slice_plan(c, a, p->S, p->kind, "oz_a", &p->nex);
+  const int none = p->shared ? kNoExp : 0;
+  const int64_t nb = M->nb();
+  ProfScope ps(c, "norms");
+  if (nb > 0) {
+    double* norm = M->d_norm;
+    int32_t* ex = const_cast<int32_t*>(p->sl.ex);
+    const double* arena = M->data();
+    if (p->kind == 2)
+      hipLaunchKernelGGL(k_norms_view_exps<2>, dim3((unsigned)nb), dim3(64), 0, c->stream, arena,
+                         (const int64_t*)M->d_off, (const int32_t*)M->d_rsz, (const int32_t*)M->d_csz, M->nrb,
+                         (const int32_t*)V->d_rsz, (const int32_t*)V->d_csz, V->nrb, nb, p->S, none, p->sl.ex_off,
+                         norm, ex);
+    else if (p->kind == 1)
+      hipLaunchKernelGGL(k_norms_view_exps<1>, dim3((unsigned)nb), dim3(64), 0, c->stream, arena,
+                         (const int64_t*)M->d_off, (const int32_t*)M->d_rsz, (const int32_t*)M->d_csz, M->nrb,
+                         (const int32_t*)V->d_rsz, (const int32_t*)V->d_csz, V->nrb, nb, p->S, none, p->sl.ex_off,
+                         norm, ex);
+    else
+      hipLaunchKernelGGL(k_norms_view_exps<0>, dim3((unsigned)nb), dim3(64), 0, c->stream, arena,
+                         (const int64_t*)M->d_off, (const int32_t*)M->d_rsz, (const int32_t*)M->d_csz, M->nrb,
+                         (const int32_t*)V->d_rsz, (const int32_t*)V->d_csz, V->nrb, nb, p->S, none, p->sl.ex_off,
+                         norm, ex);
+    SA_CHECK_HIP(hipGetLastError());
+  }
+  return OzPrepHandle(p);
+}
+
+void OzPrepDelete::operator()(OzPrep* p) const { delete p; }
 
 static constexpr int kNumOzCfgs = (int)(sizeof(kOzCfgs) / sizeof(kOzCfgs[0]));
 
@@ -979,7 +1113,7 @@ static int pick_cfg(int S, int kind, const sa_bsm* C, int tile) {
 }
 
 int64_t ozaki_multiply(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, double thresh, double alpha,
-                       const sa_opts_t& o, bool sym) {
+                       const sa_opts_t& o, bool sym, const OzPrep* prep) {
   sa_context* c = C->ctx;
   const int kind = o.slice_type;
   SA_REQUIRE(kind == SA_SLICE_FP16 || kind == SA_SLICE_BF16 || kind == SA_SLICE_INT8, "ozaki: unknown slice type %d",
@@ -995,7 +1129,15 @@ int64_t ozaki_multiply(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, double
   const sa_bsm* ra = A->base ? A->base : A;
   const sa_bsm* rb = B->base ? B->base : B;
   const bool shared = o.shared_exp != 0;
-  const Sliced sa_ = slice_operand(c, A, false, ta, S, kind, shared, "oz_a");
+  // an operand prepared with ozaki_prepare_view (its exponents computed in the
+  // pass over the matrix that also took its block norms) only gets its digits
+  Sliced sa_;
+  if (prep && prep->V == A && ta && prep->S == S && prep->kind == kind && prep->shared == shared) {
+    sa_ = prep->sl;
+    slice_run(c, slice_args(A, false, true), A->nrb, A->ncb, S, kind, shared, sa_, prep->nex, true);
+  } else {
+    sa_ = slice_operand(c, A, false, ta, S, kind, shared, "oz_a");
+  }
   const Sliced sb_ =
       (ra == rb && ta != tb && A == B) ? sa_ : slice_operand(c, B, true, tb, S, kind, shared, "oz_b");
 

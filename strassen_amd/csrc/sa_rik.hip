@@ -334,6 +334,9 @@ int sa_ri_k_step(sa_bsm_t X, sa_bsm_t B, sa_bsm_t Bp, sa_bsm_t K, int32_t n_mu, 
     if (opts_transform) ot = *opts_transform;
     if (opts_k) ok = *opts_k;
     SA_CHECK_HIP(hipSetDevice(B->ctx->device));
+    // the split K contraction takes B''s block norms in the pass over B' that
+    // also finds its slicing exponents (view blocks of at most 128 columns)
+    const bool fused_norms = ok.mode == SA_MODE_OZAKI && *std::max_element(mu_sizes, mu_sizes + n_mu) <= 128;
     // transform B' = X B
     check_mm_geometry(Bp, X, false, B, false);
     {
@@ -343,7 +346,9 @@ int sa_ri_k_step(sa_bsm_t X, sa_bsm_t B, sa_bsm_t Bp, sa_bsm_t K, int32_t n_mu, 
         ozaki_multiply(Bp, X, false, B, false, thresh, 1.0, ot);
       else
         screened_multiply(Bp, X, false, B, false, thresh, 1.0, ot.tile, nullptr, nullptr, 0, false);
-      launch_block_norms(Bp);
+      // B' block norms (the K view's: the same blocks): here for the FP64 K
+      // contraction, in the K stage's one pass over B' for the split one
+      if (!fused_norms) launch_block_norms(Bp);
     }
     // K-view of B': rows (P, i-chunk), columns mu
     const int32_t npb = Bp->nrb;
@@ -363,9 +368,14 @@ int sa_ri_k_step(sa_bsm_t X, sa_bsm_t B, sa_bsm_t Bp, sa_bsm_t K, int32_t n_mu, 
       beta_prepass(K, 0.0);
       // K = B'^T B' is symmetric: the blocks above the diagonal are computed
       // and the ones below written as their transposes
-      if (ok.mode == SA_MODE_OZAKI)
+      if (fused_norms) {
+        // B' block norms and the exponents of the K operand's vectors (the
+        // view's mu columns) in one pass over B'; the slicer then cuts digits only
+        const OzPrepHandle prep = ozaki_prepare_view(Bp, view, ok);
+        ozaki_multiply(K, view, true, view, false, thresh, 1.0, ok, true, prep.get());
+      } else if (ok.mode == SA_MODE_OZAKI) {
         ozaki_multiply(K, view, true, view, false, thresh, 1.0, ok, true);
-      else
+      } else
         screened_multiply(K, view, true, view, false, thresh, 1.0, ok.tile, nullptr, nullptr, 0, false, true);
     } catch (...) {
       sa_bsm_destroy(view);
