@@ -336,7 +336,10 @@ int sa_ri_k_step(sa_bsm_t X, sa_bsm_t B, sa_bsm_t Bp, sa_bsm_t K, int32_t n_mu, 
     SA_CHECK_HIP(hipSetDevice(B->ctx->device));
     // the split K contraction takes B''s block norms in the pass over B' that
     // also finds its slicing exponents (view blocks of at most 128 columns)
-    const bool fused_norms = ok.mode == SA_MODE_OZAKI && *std::max_element(mu_sizes, mu_sizes + n_mu) <= 128;
+    // (env SA_RIK_FUSED_NORMS=0: norms and exponents in their own passes)
+    const char* fz = std::getenv("SA_RIK_FUSED_NORMS");
+    const bool fused_norms = ok.mode == SA_MODE_OZAKI && *std::max_element(mu_sizes, mu_sizes + n_mu) <= 128 &&
+                             !(fz && std::atoi(fz) == 0);
     // transform B' = X B
     check_mm_geometry(Bp, X, false, B, false);
     {

@@ -154,3 +154,30 @@ def test_ri_k_step_split_forms_vs_oracle(kind, s_t, s_k, shared_t, shared_k, win
         ref_r = K_ref[mo[r]:mo[r + 1]]
         if np.linalg.norm(ref_r) > 0:
             assert rel_fro(K[mo[r]:mo[r + 1]], ref_r) < 1e-9
+
+
+@pytest.mark.parametrize("kind,s_k,shared_k", [("int8", 5, 1), ("int8", 5, 0), ("fp16", 4, 0), ("bf16", 5, 1)])
+def test_fused_norms_and_exponents_bit_exact(kind, s_k, shared_k, monkeypatch):
+    """The split K contraction takes B''s block norms and its slicing
+    exponents in one pass over B' (k_norms_view_exps).  With the pass split
+    (SA_RIK_FUSED_NORMS=0: the canonical block norms, then the slicer's own
+    max pass) every result must be bit-identical: task list, B', K."""
+    st = {"int8": sa.SLICE_INT8, "bf16": sa.SLICE_BF16, "fp16": sa.SLICE_FP16}[kind]
+    w = synth.rik_small()
+    dX = sa.BSM(ctx(), w.p_sizes, w.p_sizes, w.thresh)
+    X, _, _ = sa.inv_sqrt(ctx(), w.metric(), X_bsm=dX, tol=1e-14)
+    dB, _, _ = build_pair(w, X)
+    ot = sa.default_opts(mode=sa.MODE_OZAKI, slices=5, slice_type=sa.SLICE_INT8)
+    ok = sa.default_opts(mode=sa.MODE_OZAKI, slices=s_k, slice_type=st, shared_exp=shared_k)
+    out = []
+    for fused in ("1", "0"):
+        monkeypatch.setenv("SA_RIK_FUSED_NORMS", fused)
+        dBp = sa.BSM(ctx(), w.p_sizes, w.b_col_sizes, 0.0)
+        dK = sa.BSM(ctx(), w.mu_sizes, w.mu_sizes, 0.0)
+        sa.ri_k_step(dX, dB, dBp, dK, w.mu_sizes, w.i_chunks, w.thresh, ot, ok)
+        out.append((ctx().tasklist().copy(), dBp.to_dense(), dK.to_dense(), dBp.meta()[1]))
+    (t1, b1, k1, n1), (t2, b2, k2, n2) = out
+    assert np.array_equal(t1, t2)
+    assert np.array_equal(b1, b2)
+    assert np.array_equal(n1, n2), "block norms differ between the fused and the separate pass"
+    assert np.array_equal(k1, k2), "K differs between fused and separate exponent passes"
