@@ -117,9 +117,19 @@ __device__ inline int64_t xcd_remap(int64_t bid, int64_t n) {
 
 // Canonical block norm: lane l accumulates fma(x,x,acc) over e = l, l+64, ...;
 // a halving tree s_l += s_{l+off} (off = 32..1); IEEE sqrt in lane 0.
+// The loads run 8 ahead of the fma chain (8 in flight per lane); the chain
+// keeps the order e = l, l + 64, l + 128, ...
 __device__ inline double wave_block_norm(const double* __restrict__ x, int64_t n, int lane) {
   double a = 0.0;
-  for (int64_t e = lane; e < n; e += 64) a = fma(x[e], x[e], a);
+  int64_t e = lane;
+  for (; e + 7 * 64 < n; e += 8 * 64) {
+    double y[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) y[t] = x[e + 64 * t];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) a = fma(y[t], y[t], a);
+  }
+  for (; e < n; e += 64) a = fma(x[e], x[e], a);
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) {
     double y = __shfl_down(a, off, 64);

@@ -308,8 +308,7 @@ __global__ void __launch_bounds__(64) k_norms_view_exps(const double* __restrict
   double a = 0.0, mx = 0.0;
   int v = lane / L;                    // vector of element lane
   int64_t next = (int64_t)(v + 1) * L;  // first element of the following vector
-  for (int64_t e = lane; e < n; e += 64) {
-    const double y = x[e];
+  auto take = [&](int64_t e, double y) {
     a = fma(y, y, a);
     if (e >= next) {  // crossed into another vector: flush the running max
       atomicMax(&colmax[v], (unsigned long long)__double_as_longlong(mx));
@@ -318,7 +317,17 @@ __global__ void __launch_bounds__(64) k_norms_view_exps(const double* __restrict
       mx = 0.0;
     }
     mx = fmax(mx, fabs(y));
+  };
+  // loads 8 ahead of the fma chain, whose order stays e = l, l + 64, ...
+  int64_t e = lane;
+  for (; e + 7 * 64 < n; e += 8 * 64) {
+    double y[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) y[t] = x[e + 64 * t];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) take(e + 64 * t, y[t]);
   }
+  for (; e < n; e += 64) take(e, x[e]);
   if (lane < n) atomicMax(&colmax[v], (unsigned long long)__double_as_longlong(mx));
 #pragma unroll
   for (int s = 32; s >= 1; s >>= 1) a = a + __shfl_down(a, s, 64);
