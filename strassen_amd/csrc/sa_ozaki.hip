@@ -202,25 +202,40 @@ __global__ void __launch_bounds__(64) k_slice(SliceArgs s, int S, const int64_t*
   uint8_t* out = slices + sl_off[b] + (int64_t)vb * nks * 1024 + lane * 16;
   const int vl = lane & 15, kg = lane >> 4;
 
-  // T[i][kl] = x(vector v0 + i, k = KS ks + kl), zero outside the block
-  auto stage = [&](int ks) {
+  // one k-step tile T[i][kl] = x(vector v0 + i, k = KS ks + kl), zero outside
+  // the block: each lane loads NV values into registers (load) and stores them
+  // to T (put), so the next tile's loads can be in flight while a tile is cut
+  constexpr int NV = KS / 4;
+  auto load = [&](int ks, double (&r)[NV]) {
     const int k0 = ks * KS;
     if (sk == 1) {  // k contiguous: each load covers 64 / KS vectors x KS k
       constexpr int VPI = 64 / KS;
       const int kl = lane % KS, vo = lane / KS;
 #pragma unroll
-      for (int j = 0; j < 16 / VPI; ++j) {
+      for (int j = 0; j < NV; ++j) {
         const int i = j * VPI + vo, v = v0 + i, kk = k0 + kl;
-        T[i][kl] = (v < nv && kk < kd) ? xb[(int64_t)v * sv + kk] : 0.0;
+        r[j] = (v < nv && kk < kd) ? xb[(int64_t)v * sv + kk] : 0.0;
       }
     } else {  // vectors contiguous (or neither): each load covers 16 vectors x 4 k
 #pragma unroll
-      for (int j = 0; j < KS / 4; ++j) {
+      for (int j = 0; j < NV; ++j) {
         const int kl = j * 4 + kg, v = v0 + vl, kk = k0 + kl;
-        T[vl][kl] = (v < nv && kk < kd) ? xb[(int64_t)v * sv + (int64_t)kk * sk] : 0.0;
+        r[j] = (v < nv && kk < kd) ? xb[(int64_t)v * sv + (int64_t)kk * sk] : 0.0;
       }
     }
   };
+  auto put = [&](const double (&r)[NV]) {
+    if (sk == 1) {
+      constexpr int VPI = 64 / KS;
+      const int kl = lane % KS, vo = lane / KS;
+#pragma unroll
+      for (int j = 0; j < NV; ++j) T[j * VPI + vo][kl] = r[j];
+    } else {
+#pragma unroll
+      for (int j = 0; j < NV; ++j) T[vl][j * 4 + kg] = r[j];
+    }
+  };
+  double cur[NV], nxt[NV];
 
   const int64_t mask = ((int64_t)1 << BETA) - 1, half = (int64_t)1 << (BETA - 1);
   int64_t H = 0;
@@ -230,12 +245,16 @@ __global__ void __launch_bounds__(64) k_slice(SliceArgs s, int S, const int64_t*
     e = exps[ex_off[b] + v0 + vl];  // >= this block's own exponent: digits stay in range
   } else {
     double mx = 0.0;
+    load(0, cur);
     for (int ks = 0; ks < nks; ++ks) {
-      stage(ks);
+      put(cur);
       __syncthreads();
+      if (ks + 1 < nks) load(ks + 1, nxt);
 #pragma unroll
       for (int t = 0; t < EL; ++t) mx = fmax(mx, fabs(T[vl][kg * EL + t]));
       __syncthreads();
+#pragma unroll
+      for (int j = 0; j < NV; ++j) cur[j] = nxt[j];
     }
     mx = fmax(mx, __shfl_xor(mx, 16, 64));
     mx = fmax(mx, __shfl_xor(mx, 32, 64));
@@ -245,13 +264,17 @@ __global__ void __launch_bounds__(64) k_slice(SliceArgs s, int S, const int64_t*
     if (mode == 1) return;
   }
 
+  load(0, cur);
   for (int ks = 0; ks < nks; ++ks) {
-    stage(ks);
+    put(cur);
     __syncthreads();
+    if (ks + 1 < nks) load(ks + 1, nxt);
     int64_t mv[EL];
 #pragma unroll
     for (int t = 0; t < EL; ++t) mv[t] = (int64_t)rint(ldexp(T[vl][kg * EL + t], BETA * S - TS - e)) + H;
     __syncthreads();
+#pragma unroll
+    for (int j = 0; j < NV; ++j) cur[j] = nxt[j];
     for (int p = 0; p < S; ++p) {
       const int sh = BETA * (S - 1 - p);
       uint32_t w[4] = {0u, 0u, 0u, 0u};
