@@ -482,6 +482,56 @@ __global__ void k_emit_items(const GemmOut* __restrict__ outs, int64_t nouts, in
     for (int a = 0; a < tm; ++a) items[s++] = GemmItem{(int32_t)o, (int16_t)a, (int16_t)b};
 }
 
+// Work items in group-column order: the outputs of one group column (the up to
+// 4 consecutive records with the same nonzero group: C blocks of 4 block rows
+// in one block column) emit their tiles together, in chunks of CC column
+// tiles: for each chunk, each output's tiles of those columns, row tile
+// fastest.  The tiles running together on one XCD then span several block
+// rows (sharing op(B) fragments, whose product lists mostly coincide) and a
+// chunk of column tiles (sharing op(A) fragments), instead of the 3-4 row
+// tiles of one block.  Outputs with group 0 keep their own order (row tile
+// fastest).  One thread per run; start: exclusive scan of the tile counts.
+__global__ void k_emit_items_grouped(const GemmOut* __restrict__ outs, int64_t nouts, int TM, int TN, int CC,
+                                     const int64_t* __restrict__ start, GemmItem* __restrict__ items) {
+  const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= nouts) return;
+  const int32_t g = outs[o].group;
+  if (g != 0 && o > 0 && outs[o - 1].group == g) return;  // not the first of its run
+  int64_t end = o + 1;
+  if (g != 0)
+    while (end < nouts && end - o < 4 && outs[end].group == g) ++end;
+  int maxtn = 0;
+  for (int64_t q = o; q < end; ++q) maxtn = max(maxtn, (int)ceil_div(outs[q].n, TN));
+  int64_t s = start[o];
+  for (int c0 = 0; c0 < maxtn; c0 += CC)
+    for (int64_t q = o; q < end; ++q) {
+      const int tm = (int)ceil_div(outs[q].m, TM), tn = (int)ceil_div(outs[q].n, TN);
+      for (int b = c0; b < min(c0 + CC, tn); ++b)
+        for (int a = 0; a < tm; ++a) items[s++] = GemmItem{(int32_t)q, (int16_t)a, (int16_t)b};
+    }
+}
+
+// column-tile chunk of the group-column item order (env SA_ITEM_CHUNK; 0: each
+// output's tiles on their own)
+int item_chunk() {
+  static const int cc = [] {
+    const char* v = std::getenv("SA_ITEM_CHUNK");
+    return v ? std::max(0, std::atoi(v)) : 8;
+  }();
+  return cc;
+}
+
+void emit_items(sa_context* c, const GemmOut* outs, int64_t nouts, int TM, int TN, const int64_t* start,
+                GemmItem* items, bool grouped) {
+  const int cc = item_chunk();
+  if (grouped && cc > 0)
+    hipLaunchKernelGGL(k_emit_items_grouped, dim3((unsigned)ceil_div(nouts, 256)), dim3(256), 0, c->stream, outs,
+                       nouts, TM, TN, cc, start, items);
+  else
+    hipLaunchKernelGGL(k_emit_items, dim3((unsigned)ceil_div(nouts, 256)), dim3(256), 0, c->stream, outs, nouts, TM,
+                       TN, start, items);
+}
+
 // ------------------------------------------------------------- configs --
 using GemmFn = void (*)(const GemmOut*, const GemmProd*, const GemmItem*, int64_t, double);
 
@@ -588,8 +638,7 @@ static GemmPlan plan_items(sa_context* c, const GemmBatch& b, int cfg) {
                        k.tm, k.tn, cnt);
     nitems = scan_exclusive_i64(c, cnt, b.nouts);
     c->ws_gemm_items2.reserve(sizeof(GemmItem) * (size_t)(nitems + 1));
-    hipLaunchKernelGGL(k_emit_items, dim3((unsigned)ceil_div(b.nouts, 256)), dim3(256), 0, c->stream, b.outs,
-                       b.nouts, k.tm, k.tn, (const int64_t*)cnt, c->ws_gemm_items2.as<GemmItem>());
+    emit_items(c, b.outs, b.nouts, k.tm, k.tn, (const int64_t*)cnt, c->ws_gemm_items2.as<GemmItem>(), true);
     items = c->ws_gemm_items2.as<GemmItem>();
   }
   SA_REQUIRE(nitems < (int64_t)1 << 31, "too many GEMM work items");
@@ -641,8 +690,9 @@ WindowItems window_items(sa_context* c, const GemmOut* outs, const std::vector<i
   SA_REQUIRE(n < ((int64_t)1 << 31) && nitems < ((int64_t)1 << 31), "too many GEMM work items");
   DevBuf& bit = c->buf("win_items");
   bit.reserve(sizeof(GemmItem) * (size_t)(nitems + 1));
-  hipLaunchKernelGGL(k_emit_items, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, c->stream, outs, n, TM, TN,
-                     (const int64_t*)cnt, bit.as<GemmItem>());
+  // group-column order within one launch only: the windows of a windowed
+  // multiply hold records of the same C blocks
+  emit_items(c, outs, n, TM, TN, (const int64_t*)cnt, bit.as<GemmItem>(), nw1 == 2);
   SA_CHECK_HIP(hipGetLastError());
   W.items = bit.as<GemmItem>();
   return W;

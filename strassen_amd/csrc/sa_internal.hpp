@@ -303,7 +303,7 @@ struct GemmOut {
   int32_t m, n;
   int32_t p0, p1;
   int32_t acc;
-  int32_t pad;
+  int32_t group;  // 1 + group column of the C block (out_slot / 4), 0 = none: the work-item order
 };
 // op(A) is m x k, op(B) is k x n; stored blocks are column-major with
 // ld = rows of the stored block (A: m or k, B: k or n).

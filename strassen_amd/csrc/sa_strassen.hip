@@ -365,7 +365,7 @@ __global__ void k_leaf_records(const LeafProd* __restrict__ lp, Ptrs P, int32_t 
     o.p0 = (int32_t)p0;
     o.p1 = (int32_t)(p0 + h);
     o.acc = 0;
-    o.pad = 0;
+    o.group = 0;
     outs[oi] = o;
     for (int kb = 0; kb < h; ++kb) {
       const double* a = resolve(L.x, r, kb, P);
