@@ -304,6 +304,8 @@ struct GemmOut {
   int32_t p0, p1;
   int32_t acc;
   int32_t group;  // 1 + group column of the C block (out_slot / 4), 0 = none: the work-item order
+  int32_t hint;   // split-precision GEMM: 1 = take the zero-slice path (its A slices often all zero)
+  int32_t pad_;
 };
 // op(A) is m x k, op(B) is k x n; stored blocks are column-major with
 // ld = rows of the stored block (A: m or k, B: k or n).

@@ -182,6 +182,8 @@ __global__ void k_build_records(MMTables t, GemmOut* __restrict__ outs, GemmProd
   o.p1 = (int32_t)(seg + cnt);
   o.acc = fresh ? 0 : 1;
   o.group = (int32_t)(out_slot(ib, jb, t.nib) >> 2) + 1;  // group column (4 C blocks of one column), + 1
+  o.hint = 0;
+  o.pad_ = 0;
   outs[t.out_scan[out_slot(ib, jb, t.nib)]] = o;
   for (int64_t p = seg; p < seg + cnt; ++p) {
     const int kb = t.kb[p];

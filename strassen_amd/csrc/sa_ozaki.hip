@@ -38,6 +38,7 @@
 // KS with zeros, so every fragment load is one contiguous, coalesced 1 KB read.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 #include "sa_internal.hpp"
 
@@ -94,8 +95,13 @@ struct OzProd {
   int32_t nks;        // MFMA k-steps
   int32_t nvba;       // 16-row blocks of the op(A) block
   int32_t nvbb;       // 16-column blocks of the op(B) block
-  int32_t chain;      // 1: same row/column exponents as the previous product of its output
+  // bit 0: same row/column exponents as the previous product of its output
+  // (the products chain in one accumulation); bits 8-15: leading all-zero
+  // slices of the op(A) block (S: every slice is zero)
+  int32_t flags;
 };
+__host__ __device__ inline bool oz_chain(int32_t flags) { return (flags & 1) != 0; }
+__host__ __device__ inline int oz_zero_a(int32_t flags) { return (flags >> 8) & 0xff; }
 
 // --------------------------------------------------------------- slicing --
 struct SliceArgs {
@@ -174,7 +180,8 @@ __device__ inline int oz_vector_exp(double mx, int S, int none) {
 template <int KIND>
 __global__ void __launch_bounds__(64) k_slice(SliceArgs s, int S, const int64_t* __restrict__ sl_off,
                                               const int64_t* __restrict__ ex_off, int64_t ngroups,
-                                              uint8_t* __restrict__ slices, int32_t* __restrict__ exps, int mode) {
+                                              uint8_t* __restrict__ slices, int32_t* __restrict__ exps, int mode,
+                                              unsigned long long* __restrict__ nzmask) {
   constexpr int BETA = oz_beta(KIND), TS = oz_tshift(KIND);
   constexpr int KS = oz_kstep(KIND), ES = oz_esize(KIND), EL = 16 / ES;
   __shared__ double T[16][KS + 1];  // one k-step tile; the pad keeps the 16 rows on distinct banks
@@ -264,6 +271,7 @@ __global__ void __launch_bounds__(64) k_slice(SliceArgs s, int S, const int64_t*
     if (mode == 1) return;
   }
 
+  uint32_t nz = 0;  // slices in which this lane cut a nonzero digit
   load(0, cur);
   for (int ks = 0; ks < nks; ++ks) {
     put(cur);
@@ -290,8 +298,13 @@ __global__ void __launch_bounds__(64) k_slice(SliceArgs s, int S, const int64_t*
         }
       }
       *reinterpret_cast<uint4*>(out + (int64_t)p * pstride + (int64_t)ks * 1024) = make_uint4(w[0], w[1], w[2], w[3]);
+      if (w[0] | w[1] | w[2] | w[3]) nz |= 1u << p;
     }
   }
+  // the block's nonzero slices: OR over the wave, one atomic per group
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) nz |= __shfl_xor(nz, o, 64);
+  if (lane == 0 && nz) atomicOr(nzmask + b, (unsigned long long)nz);
 }
 
 // One pass over every block of a matrix M whose slicing view V (same blocks,
@@ -420,6 +433,10 @@ struct OzTables {
   const int64_t* bsl_off;
   const int32_t* bex;
   const int64_t* bex_off;
+  const int64_t* amask;  // nonzero slices of each block of op(A)'s slicing (bit q: slice q)
+  int32_t zskip;         // 0: record no zero slice (no pair is skipped)
+  int32_t zmin;          // zero-slice path from this many sixteenths of zero slice-pair products
+  int32_t slices;
   int32_t kstep;
   int32_t shared;  // operands sliced with shared exponents: the products of an output chain
 };
@@ -447,7 +464,9 @@ __global__ void k_oz_records(OzTables t, GemmOut* __restrict__ outs, OzProd* __r
   o.p1 = (int32_t)(seg + cnt);
   o.acc = fresh ? 0 : 1;
   o.group = (int32_t)(out_slot(ib, jb, t.nib) >> 2) + 1;  // group column (4 C blocks of one column), + 1
-  outs[t.out_scan[out_slot(ib, jb, t.nib)]] = o;
+  o.pad_ = 0;
+  const int S = t.slices;
+  int64_t w_all = 0, w_zero = 0;  // slice-pair k-steps of the output, and those with an all-zero A slice
   for (int64_t p = seg; p < seg + cnt; ++p) {
     const int kb = t.kb[p];
     const int64_t ai = t.ta ? (int64_t)kb + (int64_t)ib * t.a_nrb : (int64_t)ib + (int64_t)kb * t.a_nrb;
@@ -460,9 +479,18 @@ __global__ void k_oz_records(OzTables t, GemmOut* __restrict__ outs, OzProd* __r
     pr.nks = (int32_t)ceil_div(t.ksz[kb], t.kstep);
     pr.nvba = (int32_t)ceil_div(o.m, 16);
     pr.nvbb = (int32_t)ceil_div(o.n, 16);
-    pr.chain = (t.shared && p > seg) ? 1 : 0;
+    const uint32_t ma = t.zskip ? (uint32_t)(t.amask[ai] & 0xff) : 0xffu;
+    const int za = ma ? min((int)__builtin_ctz(ma), S) : S;
+    pr.flags = ((t.shared && p > seg) ? 1 : 0) | (za << 8);
     prods[p] = pr;
+    // pairs (pa, pb) with pa < za: the sum over pa < za of S - pa
+    w_all += (int64_t)pr.nks * (S * (S + 1) / 2);
+    w_zero += (int64_t)pr.nks * (za * S - za * (za - 1) / 2);
   }
+  // the kernel's zero-slice path (one branch per A slice) pays when enough
+  // of the output's slice-pair products are zero
+  o.hint = (w_zero > 0 && 16 * w_zero >= t.zmin * w_all) ? 1 : 0;
+  outs[t.out_scan[out_slot(ib, jb, t.nib)]] = o;
 }
 
 // ----------------------------------------------------------------- GEMM --
@@ -490,20 +518,6 @@ __device__ inline void oz_dispatch(int na, int nb, F&& f) {
   else f(C0{}, C0{});
 }
 
-// One workgroup of WM x WN waves (default 2 x 2) owns a (16 FM WM) x
-// (16 FN WN) output tile; each wave owns (16 FM) x (16 FN).  A stage is one MFMA k-step for all S slices of
-// the tile's op(A) rows and op(B) columns, i.e. S (2 FM + 2 FN) fragments of
-// 1 KB, double-buffered in LDS and loaded straight from global memory into
-// LDS (no register staging).  The stream of stages runs over the products of
-// the output; the loads of stage s+1 are in flight while the MFMAs of stage s
-// run, with one barrier per stage.  Fragments beyond the output block are
-// neither loaded nor multiplied: the whole stage loop is specialised on the
-// wave's live fragment counts, so each specialisation issues its MFMAs as one
-// straight-line block (a branch per fragment would put every MFMA in its own
-// basic block and serialise it behind the LDS reads).
-// Issue order of one stage's LDS reads and MFMAs: the B slices and the first
-// A slice up front, then each A slice's MFMAs with the next A slice read right
-// after the first of them, so its LDS latency hides behind the rest.
 template <int S, int NA, int NB, int PA>
 __device__ inline void oz_sched_slice() {
   constexpr int n = (S - PA) * NA * NB;
@@ -521,6 +535,23 @@ __device__ inline void oz_sched(std::integer_sequence<int, PA...>) {
   (oz_sched_slice<S, NA, NB, PA>(), ...);
 }
 
+// One workgroup of WM x WN waves (default 2 x 2) owns a (16 FM WM) x
+// (16 FN WN) output tile; each wave owns (16 FM) x (16 FN).  A stage is one MFMA k-step for all S slices of
+// the tile's op(A) rows and op(B) columns, i.e. S (2 FM + 2 FN) fragments of
+// 1 KB, double-buffered in LDS and loaded straight from global memory into
+// LDS (no register staging).  The stream of stages runs over the products of
+// the output; the loads of stage s+1 are in flight while the MFMAs of stage s
+// run, with one barrier per stage.  Fragments beyond the output block are
+// neither loaded nor multiplied: the whole stage loop is specialised on the
+// wave's live fragment counts, so each specialisation issues its MFMAs as one
+// straight-line block (a branch per fragment would put every MFMA in its own
+// basic block and serialise it behind the LDS reads).
+// Zero slices: with exponents shared along a line, a block far below its
+// line's largest has leading slices that are all zero.  The slicer records
+// each block's nonzero slices; the product record carries the op(A) block's
+// leading zero count, and outputs where enough of the slice-pair products are
+// zero (3 sixteenths by default) take a path that neither loads nor
+// multiplies those pairs (the output's hint).
 template <int S, int KIND, int FM, int FN, int WM = 2, int WN = 2>
 __global__ void __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_eu(2)))
     k_gemm_ozaki(const GemmOut* __restrict__ outs, const OzProd* __restrict__ prods,
@@ -551,30 +582,37 @@ __global__ void __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_
   const int naj = __builtin_amdgcn_readfirstlane(ncb - cb0 - wn * FN);
 
   // stage (product pr, k-step ks) into LDS buffer buf: one 1 KB fragment per
-  // wave-wide 16-byte load; fragments beyond the output block are skipped
-  auto gload = [&](const OzProd& pr, int ks, auto par) {
+  // wave-wide 16-byte load; fragments beyond the output block are skipped,
+  // and on the zero-slice path (ZS) the A slices below the product's first
+  // nonzero one and the B slices that meet only those
+  auto gload = [&](const OzProd& pr, int ks, auto par, auto zs_) {
+    constexpr bool ZS = decltype(zs_)::value;
     uint4(*As)[64] = decltype(par)::value ? As1 : As0;
     uint4(*Bs)[64] = decltype(par)::value ? Bs1 : Bs0;
     const uint4* a4 = reinterpret_cast<const uint4*>(pr.a);
     const uint4* b4 = reinterpret_cast<const uint4*>(pr.b);
     const int64_t psa = (int64_t)pr.nvba * pr.nks * 64, psb = (int64_t)pr.nvbb * pr.nks * 64;
+    const int za = ZS ? oz_zero_a(pr.flags) : 0;
 #pragma unroll
     for (int r = 0; r < LA; ++r) {
       const int c = wid + NW * r;
       const int p = c / RA, vb = rb0 + c % RA;
-      if (c < NCA && vb < nrb) lds_dma16(a4 + p * psa + ((int64_t)vb * pr.nks + ks) * 64 + lane, &As[c][0]);
+      if (c < NCA && vb < nrb && (!ZS || p >= za))
+        lds_dma16(a4 + p * psa + ((int64_t)vb * pr.nks + ks) * 64 + lane, &As[c][0]);
     }
 #pragma unroll
     for (int r = 0; r < LB; ++r) {
       const int c = wid + NW * r;
       const int p = c / RB, vb = cb0 + c % RB;
-      if (c < NCB && vb < ncb) lds_dma16(b4 + p * psb + ((int64_t)vb * pr.nks + ks) * 64 + lane, &Bs[c][0]);
+      if (c < NCB && vb < ncb && (!ZS || p + za < S))
+        lds_dma16(b4 + p * psb + ((int64_t)vb * pr.nks + ks) * 64 + lane, &Bs[c][0]);
     }
   };
 
   // the stage loop and write-back for a wave with NA x NB live fragments
-  auto run = [&](auto na_, auto nb_) {
+  auto run = [&](auto na_, auto nb_, auto zs_) {
     constexpr int NA = decltype(na_)::value, NB = decltype(nb_)::value;
+    constexpr bool ZS = decltype(zs_)::value;
     constexpr int EA = NA > 0 ? NA : 1, EB = NB > 0 ? NB : 1;  // array extents
     dbl4 acc[EA][EB];
     acc4 s32[S][EA][EB];
@@ -598,31 +636,63 @@ __global__ void __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_
 #pragma unroll
       for (int j = 0; j < NB; ++j) cc[j] = gload1(pr.eb + (cb0 + wn * FN + j) * 16 + li);
     };
-    auto compute = [&](auto par) {
+    // the MFMAs of one stage.  Plain path: the B slices and the first A slice
+    // read up front, then each A slice's MFMAs with the next A slice read
+    // right after the first of them, so its LDS latency hides behind the rest.
+    // Zero-slice path: every fragment read first, then the MFMAs of each A
+    // slice from the product's first nonzero one on (one uniform branch per
+    // slice).  A skipped pair holds an all-zero slice and adds nothing to the
+    // exact integer sums, so both paths give the same bits.
+    auto compute = [&](auto par, const OzProd& cp) {
       const uint4(*As)[64] = decltype(par)::value ? As1 : As0;
       const uint4(*Bs)[64] = decltype(par)::value ? Bs1 : Bs0;
-      uint4 bq[S][EB];
+      if constexpr (ZS) {
+        const int za = oz_zero_a(cp.flags);
+        uint4 aq[S][EA], bq[S][EB];
 #pragma unroll
-      for (int q = 0; q < S; ++q)
+        for (int q = 0; q < S; ++q) {
 #pragma unroll
-        for (int j = 0; j < NB; ++j) bq[q][j] = Bs[q * RB + wn * FN + j][lane];
-      uint4 aq[S][EA];
+          for (int j = 0; j < NB; ++j) bq[q][j] = Bs[q * RB + wn * FN + j][lane];
 #pragma unroll
-      for (int i = 0; i < NA; ++i) aq[0][i] = As[wm * FM + i][lane];
+          for (int i = 0; i < NA; ++i) aq[q][i] = As[q * RA + wm * FM + i][lane];
+        }
 #pragma unroll
-      for (int pa = 0; pa < S; ++pa) {
-        if (pa + 1 < S)
+        for (int pa = 0; pa < S; ++pa) {
+          if (pa < za) continue;
 #pragma unroll
-          for (int i = 0; i < NA; ++i) aq[pa + 1][i] = As[(pa + 1) * RA + wm * FM + i][lane];
+          for (int pb = 0; pb < S - pa; ++pb)
 #pragma unroll
-        for (int pb = 0; pb < S - pa; ++pb)
+            for (int i = 0; i < NA; ++i)
 #pragma unroll
-          for (int i = 0; i < NA; ++i)
+              for (int j = 0; j < NB; ++j)
+                s32[pa + pb][i][j] =
+                    oz_mfma(std::integral_constant<int, KIND>{}, aq[pa][i], bq[pb][j], s32[pa + pb][i][j]);
+        }
+      } else {
+        uint4 bq[S][EB];
 #pragma unroll
-            for (int j = 0; j < NB; ++j)
-              s32[pa + pb][i][j] = oz_mfma(std::integral_constant<int, KIND>{}, aq[pa][i], bq[pb][j], s32[pa + pb][i][j]);
+        for (int q = 0; q < S; ++q)
+#pragma unroll
+          for (int j = 0; j < NB; ++j) bq[q][j] = Bs[q * RB + wn * FN + j][lane];
+        uint4 aq[S][EA];
+#pragma unroll
+        for (int i = 0; i < NA; ++i) aq[0][i] = As[wm * FM + i][lane];
+#pragma unroll
+        for (int pa = 0; pa < S; ++pa) {
+          if (pa + 1 < S)
+#pragma unroll
+            for (int i = 0; i < NA; ++i) aq[pa + 1][i] = As[(pa + 1) * RA + wm * FM + i][lane];
+#pragma unroll
+          for (int pb = 0; pb < S - pa; ++pb)
+#pragma unroll
+            for (int i = 0; i < NA; ++i)
+#pragma unroll
+              for (int j = 0; j < NB; ++j)
+                s32[pa + pb][i][j] =
+                    oz_mfma(std::integral_constant<int, KIND>{}, aq[pa][i], bq[pb][j], s32[pa + pb][i][j]);
+        }
+        if constexpr (NA > 0 && NB > 0) oz_sched<S, NA, NB>(std::make_integer_sequence<int, S>{});
       }
-      if constexpr (NA > 0 && NB > 0) oz_sched<S, NA, NB>(std::make_integer_sequence<int, S>{});
     };
     // acc += sum_w s_w 2^(e_r + f_c + 2 TS - beta (w + 2)), s = 0: t = sum_w
     // s_w 2^(-beta w) by Horner steps, one scaling per element
@@ -652,7 +722,7 @@ __global__ void __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_
       OzProd pf = p + 1 < o.p1 ? gload1(prods + p + 1) : pr;
       int ks = 0;
       load_exps(pr, er, ec);
-      gload(pr, 0, I0{});
+      gload(pr, 0, I0{}, zs_);
       __syncthreads();
       int run = 0;  // k-steps in the int32 / FP32 accumulators since the last fold
       // one stage computing from buffer P while the next loads into P ^ 1;
@@ -672,13 +742,13 @@ __global__ void __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_
             if (np + 1 < o.p1) pf = gload1(prods + np + 1);
             load_exps(pn, ern, ecn);
           }
-          gload(pn, nks, std::integral_constant<int, P ^ 1>{});
+          gload(pn, nks, std::integral_constant<int, P ^ 1>{}, zs_);
         }
-        compute(par);
+        compute(par, pr);
         // fold at the end of a product unless the next one chains (same
         // exponents), and after FULL k-steps of one accumulation in any case
         ++run;
-        const bool chained = np != p && has_next && pn.chain;
+        const bool chained = np != p && has_next && oz_chain(pn.flags);
         if ((np != p && !chained) || run == FULL) {
           fold();
           run = 0;
@@ -722,7 +792,11 @@ __global__ void __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_
         }
       }
   };
-  oz_dispatch<FM, FN>(nai, naj, run);
+  // each output takes one path for all its stages (the record builder's hint)
+  if (o.hint)
+    oz_dispatch<FM, FN>(nai, naj, [&](auto a, auto b) { run(a, b, std::true_type{}); });
+  else
+    oz_dispatch<FM, FN>(nai, naj, [&](auto a, auto b) { run(a, b, std::false_type{}); });
 }
 
 // Deep-pipelined variant for large blocks: WM x WN waves (8: two per SIMD) of
@@ -980,6 +1054,7 @@ struct Sliced {
   const int64_t* sl_off;
   const int32_t* ex;
   const int64_t* ex_off;
+  const int64_t* mask;  // per block: the slices that hold a nonzero digit (bit q: slice q)
 };
 
 // slice sizes and offsets of every block, buffers reserved (one read-back of
@@ -1005,8 +1080,11 @@ static Sliced slice_plan(sa_context* c, const SliceArgs& a, int S, int kind, con
   DevBuf& bex = c->buf((t + "_ex").c_str());
   bsl.reserve((size_t)(nsl + 64));
   bex.reserve(sizeof(int32_t) * (size_t)(nex + 1));
+  DevBuf& bmk = c->buf((t + "_mask").c_str());
+  bmk.reserve(sizeof(int64_t) * (size_t)(nb + 1));
+  fill_i64(c, bmk.as<int64_t>(), nb + 1, 0);  // the digit pass ORs into it
   *nex_out = nex;
-  return Sliced{bsl.as<uint8_t>(), bso.as<int64_t>(), bex.as<int32_t>(), beo.as<int64_t>()};
+  return Sliced{bsl.as<uint8_t>(), bso.as<int64_t>(), bex.as<int32_t>(), beo.as<int64_t>(), bmk.as<int64_t>()};
 }
 
 // cut the digits (and the exponents first, unless exps_ready: then they are
@@ -1019,13 +1097,14 @@ static void slice_run(sa_context* c, const SliceArgs& a, int32_t nrb, int32_t nc
   const dim3 g((unsigned)std::max<int64_t>(1, ng)), t(64);
   uint8_t* slp = const_cast<uint8_t*>(sl.sl);
   int32_t* exp = const_cast<int32_t*>(sl.ex);
+  auto* mk = reinterpret_cast<unsigned long long*>(const_cast<int64_t*>(sl.mask));
   auto run = [&](int mode) {
     if (kind == 2)
-      hipLaunchKernelGGL(k_slice<2>, g, t, 0, c->stream, a, S, sl.sl_off, sl.ex_off, ng, slp, exp, mode);
+      hipLaunchKernelGGL(k_slice<2>, g, t, 0, c->stream, a, S, sl.sl_off, sl.ex_off, ng, slp, exp, mode, mk);
     else if (kind == 1)
-      hipLaunchKernelGGL(k_slice<1>, g, t, 0, c->stream, a, S, sl.sl_off, sl.ex_off, ng, slp, exp, mode);
+      hipLaunchKernelGGL(k_slice<1>, g, t, 0, c->stream, a, S, sl.sl_off, sl.ex_off, ng, slp, exp, mode, mk);
     else
-      hipLaunchKernelGGL(k_slice<0>, g, t, 0, c->stream, a, S, sl.sl_off, sl.ex_off, ng, slp, exp, mode);
+      hipLaunchKernelGGL(k_slice<0>, g, t, 0, c->stream, a, S, sl.sl_off, sl.ex_off, ng, slp, exp, mode, mk);
   };
   if (shared) {
     if (!exps_ready) run(1);
@@ -1144,6 +1223,22 @@ static int pick_cfg(int S, int kind, const sa_bsm* C, int tile) {
   return best;
 }
 
+// Skip the slice pairs of leading all-zero slices (env SA_OZ_ZERO_SKIP=0:
+// multiply every pair; the results are the same bits either way)
+static bool zero_skip() {
+  const char* v = std::getenv("SA_OZ_ZERO_SKIP");
+  return !(v && std::atoi(v) == 0);
+}
+
+// The zero-slice path from this many sixteenths of zero slice-pair products
+// of an output on (env SA_OZ_ZERO_MIN; 0: whenever any pair is zero).  Its
+// branches and up-front LDS reads cost about what 2-3 sixteenths of the MFMAs
+// do (RI-K step: 3 sixteenths 720 ms, 1 727 ms, 6 763 ms, never 771 ms).
+static int zero_min() {
+  const char* v = std::getenv("SA_OZ_ZERO_MIN");
+  return v ? std::max(0, std::atoi(v)) : 3;
+}
+
 int64_t ozaki_multiply(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, double thresh, double alpha,
                        const sa_opts_t& o, bool sym, const OzPrep* prep) {
   sa_context* c = C->ctx;
@@ -1186,6 +1281,10 @@ int64_t ozaki_multiply(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, double
   t.ta = ta; t.tb = tb;
   t.asl = sa_.sl; t.asl_off = sa_.sl_off; t.aex = sa_.ex; t.aex_off = sa_.ex_off;
   t.bsl = sb_.sl; t.bsl_off = sb_.sl_off; t.bex = sb_.ex; t.bex_off = sb_.ex_off;
+  t.amask = sa_.mask;
+  t.zskip = zero_skip() ? 1 : 0;
+  t.zmin = zero_min();
+  t.slices = S;
   t.kstep = oz_kstep(kind);
   t.shared = shared ? 1 : 0;
   hipLaunchKernelGGL(k_oz_records, dim3((unsigned)ceil_div(R.nc, 128)), dim3(128), 0, c->stream, t,

@@ -366,6 +366,8 @@ __global__ void k_leaf_records(const LeafProd* __restrict__ lp, Ptrs P, int32_t 
     o.p1 = (int32_t)(p0 + h);
     o.acc = 0;
     o.group = 0;
+    o.hint = 0;
+    o.pad_ = 0;
     outs[oi] = o;
     for (int kb = 0; kb < h; ++kb) {
       const double* a = resolve(L.x, r, kb, P);
@@ -739,7 +741,7 @@ static int64_t run_dense_level(sa_bsm* C, sa_bsm* A, sa_bsm* B, std::vector<Root
     std::vector<GemmOut> ho(n);
     std::vector<GemmProd> hp(n);
     for (size_t q = 0; q < n; ++q) {
-      ho[q] = GemmOut{PC + (int64_t)q * W2, (int32_t)W, (int32_t)W, (int32_t)q, (int32_t)q + 1, 0, 0};
+      ho[q] = GemmOut{PC + (int64_t)q * W2, (int32_t)W, (int32_t)W, (int32_t)q, (int32_t)q + 1, 0, 0, 0, 0};
       hp[q] = GemmProd{PA + (int64_t)q * W2, PB + (int64_t)q * W2, (int32_t)W, 0};
     }
     GemmBatch gb;

@@ -213,6 +213,40 @@ def test_deep_pipelined_kernels(tile, slices, stype, ta, tb):
     assert rel_fro(got, ref) < bound
 
 
+@pytest.mark.parametrize("stype,slices,tile", [(sa.SLICE_INT8, 5, -1), (sa.SLICE_INT8, 5, 26), (sa.SLICE_INT8, 5, 32),
+                                               (sa.SLICE_INT8, 4, -1), (sa.SLICE_FP16, 4, -1), (sa.SLICE_BF16, 5, -1)])
+@pytest.mark.parametrize("shared", [0, 1])
+@pytest.mark.parametrize("ta", [0, 1])
+@pytest.mark.parametrize("zmin", ["3", "0"])
+def test_zero_slice_skip_bit_identical(monkeypatch, stype, slices, tile, shared, ta, zmin):
+    """Inner blocks 2^-9 .. 2^-45 below the largest of their lines: with
+    shared exponents their leading slices are all zero, and outputs with
+    enough zero slice pairs (SA_OZ_ZERO_MIN sixteenths; 0: any) take the path
+    that skips them (neither loaded nor multiplied).  C is bit-identical to
+    the run that multiplies every pair (SA_OZ_ZERO_SKIP=0), on the 64x64,
+    32x32 and wide kernels, and within the bar of the FP64 oracle."""
+    monkeypatch.setenv("SA_OZ_ZERO_MIN", zmin)
+    rng = np.random.default_rng(70 + slices + 7 * shared + 3 * ta)
+    rsz, ksz, csz = [100, 37, 64], [64, 50, 64, 30, 64, 17], [80, 48]
+    kb = np.repeat(np.arange(len(ksz)), ksz)
+    a = rng.standard_normal((sum(rsz), sum(ksz))) * np.exp2(-9.0 * kb)[None, :]
+    b = rng.standard_normal((sum(ksz), sum(csz)))
+    a_, ar, ac = (a.T.copy(), ksz, rsz) if ta else (a, rsz, ksz)
+    outs = []
+    for skip in ("0", "1"):
+        monkeypatch.setenv("SA_OZ_ZERO_SKIP", skip)
+        oA, dA = pair_blocked(np.asfortranarray(a_), ar, ac, 0.0)
+        oB, dB = pair_blocked(b, ksz, csz, 0.0)
+        oC, dC = empty_pair_blocked(rsz, csz)
+        ref, got = run(oA, dA, ta, oB, dB, 0, oC, dC, 0.0,
+                       sa.default_opts(mode=sa.MODE_OZAKI, slices=slices, slice_type=stype, shared_exp=shared,
+                                       tile=tile))
+        outs.append(got)
+    assert np.array_equal(outs[0], outs[1])
+    bound = TOL if (stype == sa.SLICE_FP16 or slices >= 5) else 1e-8  # 4 INT8 slices: 31 bits
+    assert rel_fro(outs[1], ref) < bound
+
+
 @pytest.mark.parametrize("stype", [sa.SLICE_FP16, sa.SLICE_BF16, sa.SLICE_INT8])
 def test_integer_inputs_exact(stype):
     """Inputs with at most 3 significant bits per row max live in the leading
