@@ -96,12 +96,23 @@ struct OzProd {
   int32_t nvba;       // 16-row blocks of the op(A) block
   int32_t nvbb;       // 16-column blocks of the op(B) block
   // bit 0: same row/column exponents as the previous product of its output
-  // (the products chain in one accumulation); bits 8-15: leading all-zero
-  // slices of the op(A) block (S: every slice is zero)
+  // (the products chain in one accumulation)
   int32_t flags;
+  // leading all-zero slices of each 16-row group g of the op(A) block (S: all
+  // zero), byte g & 3 of za[g >> 2]; group 15 holds the least over groups 15
+  // and up.  Four words rather than a byte array: picked by selects, they
+  // stay in registers
+  uint32_t za0, za1, za2, za3;
 };
 __host__ __device__ inline bool oz_chain(int32_t flags) { return (flags & 1) != 0; }
-__host__ __device__ inline int oz_zero_a(int32_t flags) { return (flags >> 8) & 0xff; }
+// leading zero slices of 16-row group g of a product's op(A) block (shifts
+// of combined words: a select between loaded fields would become a load at
+// a computed offset and keep the record out of registers)
+__device__ inline int oz_zero_g(const OzProd& pr, int g) {
+  const int q = min(g, 15);
+  const uint64_t lo = ((uint64_t)pr.za1 << 32) | pr.za0, hi = ((uint64_t)pr.za3 << 32) | pr.za2;
+  return (int)(((q < 8 ? lo : hi) >> (8 * (q & 7))) & 0xffu);
+}
 
 // --------------------------------------------------------------- slicing --
 struct SliceArgs {
@@ -181,7 +192,7 @@ template <int KIND>
 __global__ void __launch_bounds__(64) k_slice(SliceArgs s, int S, const int64_t* __restrict__ sl_off,
                                               const int64_t* __restrict__ ex_off, int64_t ngroups,
                                               uint8_t* __restrict__ slices, int32_t* __restrict__ exps, int mode,
-                                              unsigned long long* __restrict__ nzmask) {
+                                              uint8_t* __restrict__ zg) {
   constexpr int BETA = oz_beta(KIND), TS = oz_tshift(KIND);
   constexpr int KS = oz_kstep(KIND), ES = oz_esize(KIND), EL = 16 / ES;
   __shared__ double T[16][KS + 1];  // one k-step tile; the pad keeps the 16 rows on distinct banks
@@ -301,10 +312,10 @@ __global__ void __launch_bounds__(64) k_slice(SliceArgs s, int S, const int64_t*
       if (w[0] | w[1] | w[2] | w[3]) nz |= 1u << p;
     }
   }
-  // the block's nonzero slices: OR over the wave, one atomic per group
+  // the group's leading all-zero slices (S: all zero)
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) nz |= __shfl_xor(nz, o, 64);
-  if (lane == 0 && nz) atomicOr(nzmask + b, (unsigned long long)nz);
+  if (lane == 0) zg[g] = (uint8_t)(nz ? min((int)__builtin_ctz(nz), S) : S);
 }
 
 // One pass over every block of a matrix M whose slicing view V (same blocks,
@@ -433,10 +444,11 @@ struct OzTables {
   const int64_t* bsl_off;
   const int32_t* bex;
   const int64_t* bex_off;
-  const int64_t* amask;  // nonzero slices of each block of op(A)'s slicing (bit q: slice q)
+  const uint8_t* azg;    // leading all-zero slices of each 16-vector group of op(A)'s slicing
   int32_t zskip;         // 0: record no zero slice (no pair is skipped)
   int32_t zmin;          // zero-slice path from this many sixteenths of zero slice-pair products
   int32_t slices;
+  int32_t tile_groups;   // 16-row groups per tile row of the GEMM kernel
   int32_t kstep;
   int32_t shared;  // operands sliced with shared exponents: the products of an output chain
 };
@@ -466,7 +478,9 @@ __global__ void k_oz_records(OzTables t, GemmOut* __restrict__ outs, OzProd* __r
   o.group = (int32_t)(out_slot(ib, jb, t.nib) >> 2) + 1;  // group column (4 C blocks of one column), + 1
   o.pad_ = 0;
   const int S = t.slices;
-  int64_t w_all = 0, w_zero = 0;  // slice-pair k-steps of the output, and those with an all-zero A slice
+  const int G = t.tile_groups;                  // 16-row groups per tile row
+  const int ntr = (int)ceil_div(ceil_div(o.m, 16), G);
+  int64_t w_all = 0;                            // slice-pair k-steps of a tile row
   for (int64_t p = seg; p < seg + cnt; ++p) {
     const int kb = t.kb[p];
     const int64_t ai = t.ta ? (int64_t)kb + (int64_t)ib * t.a_nrb : (int64_t)ib + (int64_t)kb * t.a_nrb;
@@ -479,17 +493,50 @@ __global__ void k_oz_records(OzTables t, GemmOut* __restrict__ outs, OzProd* __r
     pr.nks = (int32_t)ceil_div(t.ksz[kb], t.kstep);
     pr.nvba = (int32_t)ceil_div(o.m, 16);
     pr.nvbb = (int32_t)ceil_div(o.n, 16);
-    const uint32_t ma = t.zskip ? (uint32_t)(t.amask[ai] & 0xff) : 0xffu;
-    const int za = ma ? min((int)__builtin_ctz(ma), S) : S;
-    pr.flags = ((t.shared && p > seg) ? 1 : 0) | (za << 8);
+    pr.flags = (t.shared && p > seg) ? 1 : 0;
+    // the op(A) block's groups: pairs (pa, pb) with pa < z of a group are zero
+    // (the sum over pa < z of S - pa)
+    const uint8_t* zg = t.azg + t.aex_off[ai] / 16;
+    auto zgrp = [&](int g) { return g < pr.nvba ? (t.zskip ? (int)zg[g] : 0) : S; };
+    uint64_t lo = 0, hi = 0;
+    int z15 = S;
+    for (int g = 0; g < 16 || g < pr.nvba; ++g) {
+      const uint64_t z = (uint64_t)zgrp(g);
+      if (g < 8) lo |= z << (8 * g);
+      else if (g < 15) hi |= z << (8 * (g - 8));
+      else z15 = min(z15, (int)z);
+    }
+    hi |= (uint64_t)z15 << 56;
+    pr.za0 = (uint32_t)lo;
+    pr.za1 = (uint32_t)(lo >> 32);
+    pr.za2 = (uint32_t)hi;
+    pr.za3 = (uint32_t)(hi >> 32);
     prods[p] = pr;
-    // pairs (pa, pb) with pa < za: the sum over pa < za of S - pa
     w_all += (int64_t)pr.nks * (S * (S + 1) / 2);
-    w_zero += (int64_t)pr.nks * (za * S - za * (za - 1) / 2);
   }
-  // the kernel's zero-slice path (one branch per A slice) pays when enough
-  // of the output's slice-pair products are zero
-  o.hint = (w_zero > 0 && 16 * w_zero >= t.zmin * w_all) ? 1 : 0;
+  // The kernel's zero-slice path (one branch per A slice) pays for a tile row
+  // when enough of its slice-pair products are zero: the slices below the
+  // least count of the row's groups (pairs pa < z: the sum over pa < z of
+  // S - pa).  Bit 31 covers the rows from 31 on.
+  uint32_t hint = 0;
+  for (int tb = 0; tb < min(ntr, 32); ++tb) {
+    const int g0 = tb * G, g1 = tb < 31 ? g0 + G : ntr * G;
+    int64_t w_zero = 0;
+    for (int64_t p = seg; p < seg + cnt; ++p) {
+      const int kb = t.kb[p];
+      const int64_t ai = t.ta ? (int64_t)kb + (int64_t)ib * t.a_nrb : (int64_t)ib + (int64_t)kb * t.a_nrb;
+      const uint8_t* zg = t.azg + t.aex_off[ai] / 16;
+      const int nks = (int)ceil_div(t.ksz[kb], t.kstep), ng = (int)ceil_div(o.m, 16);
+      for (int g = g0; g < min(g1, ng); g += G) {
+        int z = S;
+        for (int q = g; q < min(g + G, ng); ++q) z = min(z, t.zskip ? (int)zg[q] : 0);
+        w_zero += (int64_t)nks * (z * S - z * (z - 1) / 2);
+      }
+    }
+    const int64_t rows = tb < 31 ? 1 : ntr - 31;
+    if (w_zero > 0 && 16 * w_zero >= (int64_t)t.zmin * w_all * rows) hint |= 1u << tb;
+  }
+  o.hint = (int32_t)hint;
   outs[t.out_scan[out_slot(ib, jb, t.nib)]] = o;
 }
 
@@ -548,10 +595,10 @@ __device__ inline void oz_sched(std::integer_sequence<int, PA...>) {
 // basic block and serialise it behind the LDS reads).
 // Zero slices: with exponents shared along a line, a block far below its
 // line's largest has leading slices that are all zero.  The slicer records
-// each block's nonzero slices; the product record carries the op(A) block's
-// leading zero count, and outputs where enough of the slice-pair products are
-// zero (3 sixteenths by default) take a path that neither loads nor
-// multiplies those pairs (the output's hint).
+// the leading zero slices of every 16-vector group; the product record
+// carries those of the op(A) block's row groups, and tiles where enough of
+// the slice-pair products are zero (3 sixteenths by default) take a path that
+// neither loads nor multiplies the slices below the tile rows' least count.
 template <int S, int KIND, int FM, int FN, int WM = 2, int WN = 2>
 __global__ void __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_eu(2)))
     k_gemm_ozaki(const GemmOut* __restrict__ outs, const OzProd* __restrict__ prods,
@@ -581,18 +628,30 @@ __global__ void __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_
   const int nai = __builtin_amdgcn_readfirstlane(nrb - rb0 - wm * FM);  // live fragment rows (may be <= 0)
   const int naj = __builtin_amdgcn_readfirstlane(ncb - cb0 - wn * FN);
 
+  // leading zero slices of a product's op(A) rows in the tile: the least over
+  // its live 16-row groups.  Every wave skips the same slices (a stage lasts
+  // as long as its slowest wave)
+  auto zero_tile = [&](const OzProd& pr) {
+    int z = S;
+#pragma unroll
+    for (int i = 0; i < RA; ++i)
+      if (rb0 + i < nrb) z = min(z, oz_zero_g(pr, rb0 + i));
+    return z;
+  };
+
   // stage (product pr, k-step ks) into LDS buffer buf: one 1 KB fragment per
   // wave-wide 16-byte load; fragments beyond the output block are skipped,
-  // and on the zero-slice path (ZS) the A slices below the product's first
-  // nonzero one and the B slices that meet only those
-  auto gload = [&](const OzProd& pr, int ks, auto par, auto zs_) {
+  // and on the zero-slice path (ZS) the A slices below zt, the first slice
+  // with a nonzero digit in the tile's rows, and the B slices that meet only
+  // those
+  auto gload = [&](const OzProd& pr, int ks, auto par, auto zs_, int zt) {
     constexpr bool ZS = decltype(zs_)::value;
     uint4(*As)[64] = decltype(par)::value ? As1 : As0;
     uint4(*Bs)[64] = decltype(par)::value ? Bs1 : Bs0;
     const uint4* a4 = reinterpret_cast<const uint4*>(pr.a);
     const uint4* b4 = reinterpret_cast<const uint4*>(pr.b);
     const int64_t psa = (int64_t)pr.nvba * pr.nks * 64, psb = (int64_t)pr.nvbb * pr.nks * 64;
-    const int za = ZS ? oz_zero_a(pr.flags) : 0;
+    const int za = ZS ? zt : 0;
 #pragma unroll
     for (int r = 0; r < LA; ++r) {
       const int c = wid + NW * r;
@@ -640,14 +699,14 @@ __global__ void __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_
     // read up front, then each A slice's MFMAs with the next A slice read
     // right after the first of them, so its LDS latency hides behind the rest.
     // Zero-slice path: every fragment read first, then the MFMAs of each A
-    // slice from the product's first nonzero one on (one uniform branch per
-    // slice).  A skipped pair holds an all-zero slice and adds nothing to the
+    // slice from zt, the first slice with a nonzero digit in the tile's rows
+    // of the product's op(A) block, on (one uniform branch per slice).  A skipped pair holds an all-zero slice and adds nothing to the
     // exact integer sums, so both paths give the same bits.
-    auto compute = [&](auto par, const OzProd& cp) {
+    auto compute = [&](auto par, int zt) {
       const uint4(*As)[64] = decltype(par)::value ? As1 : As0;
       const uint4(*Bs)[64] = decltype(par)::value ? Bs1 : Bs0;
       if constexpr (ZS) {
-        const int za = oz_zero_a(cp.flags);
+        const int za = zt;
         uint4 aq[S][EA], bq[S][EB];
 #pragma unroll
         for (int q = 0; q < S; ++q) {
@@ -722,7 +781,11 @@ __global__ void __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_
       OzProd pf = p + 1 < o.p1 ? gload1(prods + p + 1) : pr;
       int ks = 0;
       load_exps(pr, er, ec);
-      gload(pr, 0, I0{}, zs_);
+      // zero-slice path: the leading zero slices of the tile's rows, for the
+      // current and the next product
+      int zt = 0, ztn = 0;
+      if constexpr (ZS) zt = zero_tile(pr);
+      gload(pr, 0, I0{}, zs_, zt);
       __syncthreads();
       int run = 0;  // k-steps in the int32 / FP32 accumulators since the last fold
       // one stage computing from buffer P while the next loads into P ^ 1;
@@ -741,10 +804,11 @@ __global__ void __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_
             pn = pf;
             if (np + 1 < o.p1) pf = gload1(prods + np + 1);
             load_exps(pn, ern, ecn);
+            if constexpr (ZS) ztn = zero_tile(pn);
           }
-          gload(pn, nks, std::integral_constant<int, P ^ 1>{}, zs_);
+          gload(pn, nks, std::integral_constant<int, P ^ 1>{}, zs_, np != p ? ztn : zt);
         }
-        compute(par, pr);
+        compute(par, zt);
         // fold at the end of a product unless the next one chains (same
         // exponents), and after FULL k-steps of one accumulation in any case
         ++run;
@@ -761,6 +825,7 @@ __global__ void __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_
             for (int v = 0; v < 4; ++v) er[i][v] = ern[i][v];
 #pragma unroll
           for (int j = 0; j < NB; ++j) ec[j] = ecn[j];
+          zt = ztn;
         }
         __syncthreads();
         p = np;
@@ -792,8 +857,9 @@ __global__ void __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_
         }
       }
   };
-  // each output takes one path for all its stages (the record builder's hint)
-  if (o.hint)
+  // each tile takes one path for all its stages (the record builder's hint:
+  // bit t for tile row t, bit 31 for the rows from 31 on)
+  if ((o.hint >> min((int)item.tm, 31)) & 1)
     oz_dispatch<FM, FN>(nai, naj, [&](auto a, auto b) { run(a, b, std::true_type{}); });
   else
     oz_dispatch<FM, FN>(nai, naj, [&](auto a, auto b) { run(a, b, std::false_type{}); });
@@ -1054,7 +1120,7 @@ struct Sliced {
   const int64_t* sl_off;
   const int32_t* ex;
   const int64_t* ex_off;
-  const int64_t* mask;  // per block: the slices that hold a nonzero digit (bit q: slice q)
+  const uint8_t* zg;  // per 16-vector group (numbered as the slicer's groups): leading all-zero slices
 };
 
 // slice sizes and offsets of every block, buffers reserved (one read-back of
@@ -1080,11 +1146,10 @@ static Sliced slice_plan(sa_context* c, const SliceArgs& a, int S, int kind, con
   DevBuf& bex = c->buf((t + "_ex").c_str());
   bsl.reserve((size_t)(nsl + 64));
   bex.reserve(sizeof(int32_t) * (size_t)(nex + 1));
-  DevBuf& bmk = c->buf((t + "_mask").c_str());
-  bmk.reserve(sizeof(int64_t) * (size_t)(nb + 1));
-  fill_i64(c, bmk.as<int64_t>(), nb + 1, 0);  // the digit pass ORs into it
+  DevBuf& bzg = c->buf((t + "_zg").c_str());
+  bzg.reserve((size_t)(nex / 16 + 16));
   *nex_out = nex;
-  return Sliced{bsl.as<uint8_t>(), bso.as<int64_t>(), bex.as<int32_t>(), beo.as<int64_t>(), bmk.as<int64_t>()};
+  return Sliced{bsl.as<uint8_t>(), bso.as<int64_t>(), bex.as<int32_t>(), beo.as<int64_t>(), bzg.as<uint8_t>()};
 }
 
 // cut the digits (and the exponents first, unless exps_ready: then they are
@@ -1097,7 +1162,7 @@ static void slice_run(sa_context* c, const SliceArgs& a, int32_t nrb, int32_t nc
   const dim3 g((unsigned)std::max<int64_t>(1, ng)), t(64);
   uint8_t* slp = const_cast<uint8_t*>(sl.sl);
   int32_t* exp = const_cast<int32_t*>(sl.ex);
-  auto* mk = reinterpret_cast<unsigned long long*>(const_cast<int64_t*>(sl.mask));
+  auto* mk = const_cast<uint8_t*>(sl.zg);
   auto run = [&](int mode) {
     if (kind == 2)
       hipLaunchKernelGGL(k_slice<2>, g, t, 0, c->stream, a, S, sl.sl_off, sl.ex_off, ng, slp, exp, mode, mk);
@@ -1281,21 +1346,22 @@ int64_t ozaki_multiply(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, double
   t.ta = ta; t.tb = tb;
   t.asl = sa_.sl; t.asl_off = sa_.sl_off; t.aex = sa_.ex; t.aex_off = sa_.ex_off;
   t.bsl = sb_.sl; t.bsl_off = sb_.sl_off; t.bex = sb_.ex; t.bex_off = sb_.ex_off;
-  t.amask = sa_.mask;
+  t.azg = sa_.zg;
   t.zskip = zero_skip() ? 1 : 0;
   t.zmin = zero_min();
   t.slices = S;
   t.kstep = oz_kstep(kind);
   t.shared = shared ? 1 : 0;
+  const int cf = pick_cfg(S, kind, C, o.tile);  // a forced id that runs other slices is ignored
+  const OzCfg& k = kOzCfgs[cf];
+  const int TM = 32 * k.fm, TN = 32 * k.fn;
+  t.tile_groups = TM / 16;
   hipLaunchKernelGGL(k_oz_records, dim3((unsigned)ceil_div(R.nc, 128)), dim3(128), 0, c->stream, t,
                      bouts.as<GemmOut>(), bprods.as<OzProd>());
 
   int32_t mm = 0, mn = 0;
   for (int32_t v : C->rsz) mm = std::max(mm, v);
   for (int32_t v : C->csz) mn = std::max(mn, v);
-  const int cf = pick_cfg(S, kind, C, o.tile);  // a forced id that runs other slices is ignored
-  const OzCfg& k = kOzCfgs[cf];
-  const int TM = 32 * k.fm, TN = 32 * k.fn;
   const KWindows W = plan_kwindows(c, A, ta, B, tb, R, bouts.as<GemmOut>());
   c->last_kwindows = W.nwin;
   // work items of all windows planned together (one read-back); the window
