@@ -304,10 +304,13 @@ def bench_rik(args):
         def stepmix():
             sa.ri_k_step(dX, dB, dBp, dK, w.mu_sizes, w.i_chunks, w.thresh, oz_t, oz_k)
 
+        ctx.slice_pairs()  # reset the slice-pair counters
         for _ in range(args.warmup):
             stepmix()
         ms = timed(ctx, stepmix, args.steps)
         log(f"split-precision step ({name}): {ms:.1f} ms")
+        n_mult, n_skip = ctx.slice_pairs()
+        skip_frac = n_skip / max(1, n_mult + n_skip)
         goz_ms, goz_n = ctx.profile("gemm_ozaki")
         prof = {k: round(ctx.profile(k)[0] / args.steps, 3) for k in ("gemm_ozaki", "ozaki_slice", "screen", "norms",
                                                                        "rik_transform", "rik_k")}
@@ -315,7 +318,9 @@ def bench_rik(args):
         err = rel(K, K64)
         rows_err = [rel(K[mo[r]:mo[r + 1]], K64[mo[r]:mo[r + 1]]) for r in range(n_mu)]
         bp_sample[name] = [dBp.blocks(g, g + 1, 0, n_ic) for g in samp_pg]
-        pairs = s_t * (s_t + 1) // 2 * f_tr + s_k * (s_k + 1) // 2 * f_k_exec
+        # slice-pair products of the step, less those the kernel skips as zero
+        # (an all-zero op(A) slice in the tile's rows), i.e. executed MFMA work
+        pairs = (s_t * (s_t + 1) // 2 * f_tr + s_k * (s_k + 1) // 2 * f_k_exec) * (1.0 - skip_frac)
         peak = PEAK_INT8 if kind.startswith("int8") else PEAK_FP16
         ach = pairs / (goz_ms / args.steps * 1e-3) / 1e12
         results[name] = {"slice_type": kind.rstrip("s"), "slices_transform": s_t, "slices_k": s_k,
@@ -325,6 +330,7 @@ def bench_rik(args):
                          "rel_err_K_per_block_row": {"max": max(rows_err), "median": float(np.median(rows_err))},
                          "meets_1e-10": bool(err <= 1e-10), "ms_per_step": prof,
                          "lowp_tops": round(ach, 3), "lowp_peak": peak, "lowp_frac": round(ach / peak, 4),
+                         "slice_pairs_skipped_as_zero": round(skip_frac, 4),
                          "kernel_avg_ms": round(goz_ms / max(goz_n, 1), 3), "kernel_launches_per_step": goz_n // args.steps}
     ctx.set_profiling(False)
     ok = [k for k, v in results.items() if v["meets_1e-10"]]
@@ -363,12 +369,15 @@ def bench_rik(args):
         "roofline": {"bound": "mfma", "achieved": H["lowp_tops"], "peak": H["lowp_peak"],
                      "unit": "TOP/s" if H["slice_type"].startswith("int8") else "TFLOP/s",
                      "frac": H["lowp_frac"], "kernel": "k_gemm_ozaki",
+                     "slice_pairs_skipped_as_zero": H["slice_pairs_skipped_as_zero"],
+                     "frac_counting_skipped_pairs": round(H["lowp_frac"] / max(1e-9, 1.0 - H["slice_pairs_skipped_as_zero"]), 4),
                      **profiled_traffic(RIK_PROFILES.get(head) if args.n_occ == 200 else None),
                      "kernel_avg_ms": H["kernel_avg_ms"],
                      "note": "achieved = executed low-precision slice-product operations (slice pairs x algorithmic "
-                             "flop of the computed products: the transform's and K's upper triangle) / k_gemm_ozaki "
-                             "time over the timed steps; the FP64 path's k_gemm_fp64 is in ri_k.fp64_gemm_tflops "
-                             "(peak 78.6)"},
+                             "flop of the computed products: the transform's and K's upper triangle, less the "
+                             "fraction of slice pairs the kernel skips as zero, slice_pairs_skipped_as_zero) / "
+                             "k_gemm_ozaki time over the timed steps; the FP64 path's k_gemm_fp64 is in "
+                             "ri_k.fp64_gemm_tflops (peak 78.6)"},
     }
     if not args.no_cpu:
         import oracle

@@ -99,6 +99,13 @@ int sa_ctx_kwindows(sa_ctx_t ctx, int32_t* nwin);
  * captured on every call and replayed through one kept executable.  No
  * counterpart in the reference, which has no device graphs. */
 int sa_ctx_graph_stats(sa_ctx_t ctx, int64_t* instantiated, int64_t* updated, int64_t* launched);
+/* Split-precision slice-pair work of the context's multiplies since the last
+ * call (then reset): slice-pair MFMA k-steps times GEMM tiles multiplied, and
+ * those skipped because the op(A) slice of the pair is all zero in the tile's
+ * rows (a block far below its line's shared exponent; the skipped pairs add
+ * nothing, results are unchanged).  No counterpart in the reference, which
+ * multiplies in FP64 (blocksparsematrix.hpp:391-404). */
+int sa_ctx_slice_pairs(sa_ctx_t ctx, int64_t* multiplied, int64_t* skipped);
 /* Wall time of the context stream between two sa_ctx_mark calls (ms). */
 int sa_ctx_mark(sa_ctx_t ctx, int slot);
 int sa_ctx_elapsed(sa_ctx_t ctx, int slot0, int slot1, float* ms);

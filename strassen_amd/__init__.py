@@ -65,6 +65,7 @@ ABI = {
     "sa_ctx_gemm_config": (INT, [P, P, C.c_char_p, I32]),
     "sa_ctx_kwindows": (INT, [P, P]),
     "sa_ctx_graph_stats": (INT, [P, P, P, P]),
+    "sa_ctx_slice_pairs": (INT, [P, P, P]),
     "sa_mlp_info": (INT, [P, P, P]),
     "sa_mlp_set": (INT, [P, P, I64, P, P]),
     "sa_mlp_get": (INT, [P, P, P, P]),
@@ -213,6 +214,13 @@ class Context:
         v = [C.c_int64(0) for _ in range(3)]
         _check(lib().sa_ctx_graph_stats(self.h, *(C.byref(x) for x in v)))
         return dict(zip(("instantiated", "updated", "launched"), (x.value for x in v)))
+
+    def slice_pairs(self) -> tuple:
+        """Split-precision slice-pair work since the last call (then reset):
+        (multiplied, skipped as zero), in slice-pair k-steps x GEMM tiles."""
+        v = [C.c_int64(0) for _ in range(2)]
+        _check(lib().sa_ctx_slice_pairs(self.h, *(C.byref(x) for x in v)))
+        return v[0].value, v[1].value
 
     def mark(self, slot: int):
         _check(lib().sa_ctx_mark(self.h, slot))

@@ -479,6 +479,21 @@ int sa_ctx_kwindows(sa_ctx_t c, int32_t* nwin) {
   });
 }
 
+int sa_ctx_slice_pairs(sa_ctx_t c, int64_t* multiplied, int64_t* skipped) {
+  return guard([&] {
+    SA_REQUIRE(c && multiplied && skipped, "null argument");
+    int64_t h[2] = {0, 0};
+    if (c->oz_pairs_ready) {
+      DevBuf& b = c->buf("oz_pairs");
+      SA_CHECK_HIP(hipMemcpyAsync(h, b.p, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+      SA_CHECK_HIP(hipStreamSynchronize(c->stream));
+      fill_i64(c, b.as<int64_t>(), 2, 0);
+    }
+    *multiplied = h[0];
+    *skipped = h[1];
+  });
+}
+
 int sa_ctx_mark(sa_ctx_t c, int slot) {
   return guard([&] {
     SA_REQUIRE(c && slot >= 0 && slot < 16, "bad mark slot");

@@ -198,6 +198,7 @@ struct sa_context {
   bool quiet = false;     // no profiling events (allocation-free calls)
   int32_t last_gemm_cfg = -1;
   int32_t last_kwindows = 1;  // k-windows of the last screened multiply
+  bool oz_pairs_ready = false;  // buffer "oz_pairs" (split-precision slice pairs run / skipped) zeroed
   // executable graph of the Strassen execution phase, kept between calls:
   // each call's capture is applied to it (hipGraphExecUpdate) and it is
   // launched again; a capture of another shape instantiates it anew

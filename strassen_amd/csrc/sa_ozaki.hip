@@ -449,6 +449,8 @@ struct OzTables {
   int32_t zmin;          // zero-slice path from this many sixteenths of zero slice-pair products
   int32_t slices;
   int32_t tile_groups;   // 16-row groups per tile row of the GEMM kernel
+  int32_t tile_n;        // columns per tile of the GEMM kernel
+  unsigned long long* pairs;  // += slice-pair k-steps x tiles: [0] multiplied, [1] skipped as zero
   int32_t kstep;
   int32_t shared;  // operands sliced with shared exponents: the products of an output chain
 };
@@ -519,6 +521,7 @@ __global__ void k_oz_records(OzTables t, GemmOut* __restrict__ outs, OzProd* __r
   // least count of the row's groups (pairs pa < z: the sum over pa < z of
   // S - pa).  Bit 31 covers the rows from 31 on.
   uint32_t hint = 0;
+  int64_t skipped = 0;
   for (int tb = 0; tb < min(ntr, 32); ++tb) {
     const int g0 = tb * G, g1 = tb < 31 ? g0 + G : ntr * G;
     int64_t w_zero = 0;
@@ -534,9 +537,15 @@ __global__ void k_oz_records(OzTables t, GemmOut* __restrict__ outs, OzProd* __r
       }
     }
     const int64_t rows = tb < 31 ? 1 : ntr - 31;
-    if (w_zero > 0 && 16 * w_zero >= (int64_t)t.zmin * w_all * rows) hint |= 1u << tb;
+    if (w_zero > 0 && 16 * w_zero >= (int64_t)t.zmin * w_all * rows) {
+      hint |= 1u << tb;
+      skipped += w_zero;
+    }
   }
   o.hint = (int32_t)hint;
+  const int64_t tcols = ceil_div(o.n, t.tile_n);
+  atomicAdd(t.pairs, (unsigned long long)((w_all * ntr - skipped) * tcols));
+  atomicAdd(t.pairs + 1, (unsigned long long)(skipped * tcols));
   outs[t.out_scan[out_slot(ib, jb, t.nib)]] = o;
 }
 
@@ -1356,6 +1365,14 @@ int64_t ozaki_multiply(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, double
   const OzCfg& k = kOzCfgs[cf];
   const int TM = 32 * k.fm, TN = 32 * k.fn;
   t.tile_groups = TM / 16;
+  t.tile_n = TN;
+  DevBuf& bpairs = c->buf("oz_pairs");
+  if (!c->oz_pairs_ready) {
+    bpairs.reserve(2 * sizeof(int64_t));
+    fill_i64(c, bpairs.as<int64_t>(), 2, 0);
+    c->oz_pairs_ready = true;
+  }
+  t.pairs = bpairs.as<unsigned long long>();
   hipLaunchKernelGGL(k_oz_records, dim3((unsigned)ceil_div(R.nc, 128)), dim3(128), 0, c->stream, t,
                      bouts.as<GemmOut>(), bprods.as<OzProd>());
 

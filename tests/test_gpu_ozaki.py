@@ -247,6 +247,32 @@ def test_zero_slice_skip_bit_identical(monkeypatch, stype, slices, tile, shared,
     assert rel_fro(outs[1], ref) < bound
 
 
+def test_zero_slice_pair_counts(monkeypatch):
+    """The context's slice-pair counters: no pair is skipped without shared
+    exponents or with skipping switched off, some are with shared exponents
+    on decaying blocks, and multiplied + skipped is the same work either way."""
+    rng = np.random.default_rng(77)
+    rsz, ksz, csz = [100, 64], [64, 50, 64, 30, 64], [80, 48]
+    kb = np.repeat(np.arange(len(ksz)), ksz)
+    a = rng.standard_normal((sum(rsz), sum(ksz))) * np.exp2(-9.0 * kb)[None, :]
+    b = rng.standard_normal((sum(ksz), sum(csz)))
+    counts = {}
+    for shared, skip in ((0, "1"), (1, "0"), (1, "1")):
+        monkeypatch.setenv("SA_OZ_ZERO_SKIP", skip)
+        monkeypatch.setenv("SA_OZ_ZERO_MIN", "0")
+        _, dA = pair_blocked(a, rsz, ksz, 0.0)
+        _, dB = pair_blocked(b, ksz, csz, 0.0)
+        _, dC = empty_pair_blocked(rsz, csz)
+        ctx().slice_pairs()
+        sa.matmult(dC, dA, 0, dB, 0, 0.0, 1.0, 0.0,
+                   sa.default_opts(mode=sa.MODE_OZAKI, slices=5, slice_type=sa.SLICE_INT8, shared_exp=shared))
+        counts[(shared, skip)] = ctx().slice_pairs()
+    assert counts[(0, "1")][1] == 0 and counts[(1, "0")][1] == 0
+    assert counts[(1, "1")][1] > 0
+    assert sum(counts[(1, "1")]) == sum(counts[(1, "0")]) == sum(counts[(0, "1")]) > 0
+    assert ctx().slice_pairs() == (0, 0)
+
+
 @pytest.mark.parametrize("stype", [sa.SLICE_FP16, sa.SLICE_BF16, sa.SLICE_INT8])
 def test_integer_inputs_exact(stype):
     """Inputs with at most 3 significant bits per row max live in the leading
