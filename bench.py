@@ -240,7 +240,7 @@ def bench_rik(args):
     # suffix "ss": shared exponents in the K contraction as well
     kinds = {"fp16": sa.SLICE_FP16, "bf16": sa.SLICE_BF16, "int8": sa.SLICE_INT8,
              "fp16s": sa.SLICE_FP16, "bf16s": sa.SLICE_BF16, "int8s": sa.SLICE_INT8,
-             "int8ss": sa.SLICE_INT8}
+             "fp16ss": sa.SLICE_FP16, "bf16ss": sa.SLICE_BF16, "int8ss": sa.SLICE_INT8}
     forms = []
     for f in args.forms.split(","):
         parts = f.split(":")
