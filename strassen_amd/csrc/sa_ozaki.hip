@@ -659,7 +659,9 @@ __global__ void __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_
     uint4(*Bs)[64] = decltype(par)::value ? Bs1 : Bs0;
     const uint4* a4 = reinterpret_cast<const uint4*>(pr.a);
     const uint4* b4 = reinterpret_cast<const uint4*>(pr.b);
-    const int64_t psa = (int64_t)pr.nvba * pr.nks * 64, psb = (int64_t)pr.nvbb * pr.nks * 64;
+    // offsets within one block's slices, in 16-byte units: below 2^31 (the
+    // host rejects larger blocks), so the per-stage address arithmetic is 32-bit
+    const int32_t psa = pr.nvba * pr.nks * 64, psb = pr.nvbb * pr.nks * 64;
     const int za = ZS ? zt : 0;
     if constexpr (ZS && NW % RA == 0 && NW % RB == 0 && NCA % NW == 0 && NCB % NW == 0) {
       // wave wid's fragments: one 16-row (16-column) group, wid % RA (% RB),
@@ -671,7 +673,7 @@ __global__ void __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_
       // transform gained 3 %
       const int vba = rb0 + wid % RA, vbb = cb0 + wid % RB;
       if (vba < nrb) {
-        const uint4* q = a4 + (wid / RA) * psa + ((int64_t)vba * pr.nks + ks) * 64;  // wave-uniform
+        const uint4* q = a4 + ((wid / RA) * psa + (vba * pr.nks + ks) * 64);  // wave-uniform
 #pragma unroll
         for (int r = 0; r < LA; ++r) {
           const int c = wid + NW * r;
@@ -680,7 +682,7 @@ __global__ void __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_
         }
       }
       if (vbb < ncb) {
-        const uint4* q = b4 + (wid / RB) * psb + ((int64_t)vbb * pr.nks + ks) * 64;
+        const uint4* q = b4 + ((wid / RB) * psb + (vbb * pr.nks + ks) * 64);
 #pragma unroll
         for (int r = 0; r < LB; ++r) {
           const int c = wid + NW * r;
@@ -694,14 +696,14 @@ __global__ void __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_
         const int c = wid + NW * r;
         const int p = c / RA, vb = rb0 + c % RA;
         if (c < NCA && vb < nrb && (!ZS || p >= za))
-          lds_dma16(a4 + p * psa + ((int64_t)vb * pr.nks + ks) * 64 + lane, &As[c][0]);
+          lds_dma16(a4 + (p * psa + (vb * pr.nks + ks) * 64) + lane, &As[c][0]);
       }
 #pragma unroll
       for (int r = 0; r < LB; ++r) {
         const int c = wid + NW * r;
         const int p = c / RB, vb = cb0 + c % RB;
         if (c < NCB && vb < ncb && (!ZS || p + za < S))
-          lds_dma16(b4 + p * psb + ((int64_t)vb * pr.nks + ks) * 64 + lane, &Bs[c][0]);
+          lds_dma16(b4 + (p * psb + (vb * pr.nks + ks) * 64) + lane, &Bs[c][0]);
       }
     }
   };
@@ -1342,6 +1344,20 @@ static int zero_min() {
   return v ? std::max(0, std::atoi(v)) : 3;
 }
 
+// The split-precision GEMM addresses a block's slices with 32-bit offsets in
+// 16-byte units: S slices of ceil(rows / 16) x ceil(k / kstep) fragments of 64
+// units, in either orientation, must stay below 2^31 (blocks up to ~80,000 on
+// a side at 5 INT8 slices).
+static void require_slice_offsets_32bit(const sa_bsm* X, int S, int kind) {
+  int64_t mr = 0, mc = 0;
+  for (int32_t r : X->rsz) mr = std::max<int64_t>(mr, r);
+  for (int32_t cc : X->csz) mc = std::max<int64_t>(mc, cc);
+  const int64_t ks = oz_kstep(kind);
+  const int64_t frags = std::max(ceil_div(mr, 16) * ceil_div(mc, ks), ceil_div(mc, 16) * ceil_div(mr, ks));
+  SA_REQUIRE((int64_t)S * frags * 64 < ((int64_t)1 << 31),
+             "ozaki: blocks of up to %lld x %lld are too large for split precision", (long long)mr, (long long)mc);
+}
+
 int64_t ozaki_multiply(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, double thresh, double alpha,
                        const sa_opts_t& o, bool sym, const OzPrep* prep) {
   sa_context* c = C->ctx;
@@ -1349,6 +1365,8 @@ int64_t ozaki_multiply(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, double
   SA_REQUIRE(kind == SA_SLICE_FP16 || kind == SA_SLICE_BF16 || kind == SA_SLICE_INT8, "ozaki: unknown slice type %d",
              kind);
   const int S = o.slices;
+  require_slice_offsets_32bit(A, S, kind);
+  require_slice_offsets_32bit(B, S, kind);
   const int cfg = pick_cfg(S, kind, C, -1);
   SA_REQUIRE(cfg >= 0, "ozaki: %d %s slices not supported (FP16: 3..5, BF16: 4..6, INT8: 4..6)", S,
              kind == 2 ? "INT8" : kind == 1 ? "BF16" : "FP16");

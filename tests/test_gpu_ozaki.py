@@ -307,6 +307,17 @@ def test_unsupported_slice_count_rejected():
         sa.matmult(dC, dA, 0, dB, 0, 0.0, 1.0, 0.0, sa.default_opts(mode=sa.MODE_OZAKI, slices=9))
 
 
+def test_oversized_blocks_rejected():
+    """The split GEMM addresses a block's slices with 32-bit offsets in 16-byte
+    units: a 90000 x 90000 block at 5 INT8 slices (5 * 5625 * 1407 * 64 units,
+    above 2^31) is refused before any work; the operands are empty."""
+    big = [90000]
+    dA, dB, dC = (sa.BSM(ctx(), big, big, 0.0) for _ in range(3))
+    with pytest.raises(sa.SAError, match="too large"):
+        sa.matmult(dC, dA, 0, dB, 0, 0.0, 1.0, 0.0,
+                   sa.default_opts(mode=sa.MODE_OZAKI, slices=5, slice_type=sa.SLICE_INT8))
+
+
 def test_kwindows_split_precision(monkeypatch):
     """k-windows under the split-precision kernel: same task list, same bar."""
     cfg = synth.band_workload(n=1024, bs=64)
