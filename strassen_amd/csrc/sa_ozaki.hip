@@ -667,10 +667,9 @@ __global__ void __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_
       // wave wid's fragments: one 16-row (16-column) group, wid % RA (% RB),
       // and the slices wid / RA + (NW / RA) r: one address per stage plus a
       // fixed slice stride, and one bounds branch per operand (the scalar
-      // work per load stays a 64-bit add).  Only on the zero-slice path: on
-      // the plain path the K contraction of the RI-K step ran 6 % slower with
-      // its loads issued back to back (348 -> 370 ms per step) while the
-      // transform gained 3 %
+      // work per load stays a 64-bit add).  Only on the zero-slice path: with
+      // the plain path's loads issued this way (in either operand order, or
+      // one per MFMA) the K contraction of the RI-K step ran 2-10 % slower
       const int vba = rb0 + wid % RA, vbb = cb0 + wid % RB;
       if (vba < nrb) {
         const uint4* q = a4 + ((wid / RA) * psa + (vba * pr.nks + ks) * 64);  // wave-uniform
