@@ -1353,8 +1353,9 @@ static void require_slice_offsets_32bit(const sa_bsm* X, int S, int kind) {
   for (int32_t cc : X->csz) mc = std::max<int64_t>(mc, cc);
   const int64_t ks = oz_kstep(kind);
   const int64_t frags = std::max(ceil_div(mr, 16) * ceil_div(mc, ks), ceil_div(mc, 16) * ceil_div(mr, ks));
-  SA_REQUIRE((int64_t)S * frags * 64 < ((int64_t)1 << 31),
-             "ozaki: blocks of up to %lld x %lld are too large for split precision", (long long)mr, (long long)mc);
+  if ((int64_t)S * frags * 64 >= ((int64_t)1 << 31))
+    SA_UNSUPPORTED("ozaki: blocks of up to %lld x %lld are too large for split precision", (long long)mr,
+                   (long long)mc);
 }
 
 int64_t ozaki_multiply(sa_bsm* C, sa_bsm* A, bool ta, sa_bsm* B, bool tb, double thresh, double alpha,

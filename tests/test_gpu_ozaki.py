@@ -313,9 +313,10 @@ def test_oversized_blocks_rejected():
     above 2^31) is refused before any work; the operands are empty."""
     big = [90000]
     dA, dB, dC = (sa.BSM(ctx(), big, big, 0.0) for _ in range(3))
-    with pytest.raises(sa.SAError, match="too large"):
+    with pytest.raises(sa.SAError, match="too large") as err:
         sa.matmult(dC, dA, 0, dB, 0, 0.0, 1.0, 0.0,
                    sa.default_opts(mode=sa.MODE_OZAKI, slices=5, slice_type=sa.SLICE_INT8))
+    assert err.value.code == sa.ERR_UNSUPPORTED
 
 
 def test_kwindows_split_precision(monkeypatch):
