@@ -676,7 +676,7 @@ __global__ void __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_
 #pragma unroll
         for (int r = 0; r < LA; ++r) {
           const int c = wid + NW * r;
-          if (!ZS || wid / RA + (NW / RA) * r >= za) lds_dma16(q + lane, &As[c][0]);
+          if (wid / RA + (NW / RA) * r >= za) lds_dma16(q + lane, &As[c][0]);
           q += (NW / RA) * psa;
         }
       }
@@ -685,7 +685,7 @@ __global__ void __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_
 #pragma unroll
         for (int r = 0; r < LB; ++r) {
           const int c = wid + NW * r;
-          if (!ZS || wid / RB + (NW / RB) * r + za < S) lds_dma16(q + lane, &Bs[c][0]);
+          if (wid / RB + (NW / RB) * r + za < S) lds_dma16(q + lane, &Bs[c][0]);
           q += (NW / RB) * psb;
         }
       }
